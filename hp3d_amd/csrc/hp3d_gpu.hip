@@ -1,0 +1,945 @@
+// hp3d_gpu.hip -- MI355X (gfx950) element computation for hp3D hexahedra.
+//
+// Kernels
+//   elem_kernel   one workgroup per (element, 128x128 block of K): 1D factor
+//                 tables and per-point geometry in LDS, A = B^T(geometry)
+//                 streamed through LDS in chunks of quadrature points, and
+//                 K += A A^T on MFMA (v_mfma_f64_16x16x4f64 or
+//                 v_mfma_f32_16x16x4f32), upper-triangle tiles only, written
+//                 out in both triangles with coalesced column segments.
+//   stc_kernel    one workgroup per element: Cholesky of the bubble block,
+//                 X = Abb^-1 [Abi | Bb], Aii - Aib X and Bi - Aib Xb, FP64.
+//   stc_bwd_kernel  xb = BSchur - ASchur xi.
+//
+// The reference computes the same quantities per element on one CPU thread:
+// POISSON/GALERKIN/elem_opt.F90:88-137 (A_TEST fill + DSYRK), the real form of
+// MAXWELL/GALERKIN/elem_opt.F90:88-149, modules/stc.F90:182-416.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <mutex>
+
+#include "../../include/hp3d_gpu.h"
+#include "hp3d_tables.h"
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+namespace hp3d {
+
+constexpr int WG = 256;        // threads per workgroup (4 waves)
+constexpr int BR = 128;        // rows of K per block (8 MFMA tiles)
+constexpr int KC = 24;         // A columns per chunk (8 points H1, 4 points H(curl))
+constexpr int LDA = 2 * BR + 4; // LDS stride of one A column
+constexpr int NPT = 8;         // padded max Gauss points per direction
+constexpr int MAXQ = (MAXP + 1) * (MAXP + 1) * (MAXP + 1);
+constexpr int MAXN_H1 = (MAXP + 1) * (MAXP + 1) * (MAXP + 1);
+constexpr int MAXN_E = 3 * MAXP * (MAXP + 1) * (MAXP + 1);
+constexpr int STC_LDS_NB = 88; // bubble blocks up to this size are factored in LDS
+
+__constant__ double c_gx[10][10] = HP3D_GAUSS_X;
+__constant__ double c_gw[10][10] = HP3D_GAUSS_W;
+
+// ------------------------------------------------------------------ helpers
+
+// 1D factor tables for one direction at the n Gauss points (values, d/dxi)
+__device__ void fill_tables(int n, double *tv, double *td)
+{
+    // tv/td laid out [NTAB][NPT]
+    for (int q = threadIdx.x % 32; q < n; q += 32) {
+        double x = 0.5 * (1.0 + c_gx[n - 1][q]);
+        tv[0 * NPT + q] = 1.0 - x;
+        td[0 * NPT + q] = -1.0;
+        tv[1 * NPT + q] = x;
+        td[1 * NPT + q] = 1.0;
+        for (int flip = 0; flip < 2; ++flip) {
+            // integrated Legendre phi_i(s) = (P_i(2s-1) - P_{i-2}(2s-1)) / (4i-2),
+            // d phi_i / ds = P_{i-1}(2s-1)   (PolyILegendre, Idec = .true.)
+            double s = flip ? 1.0 - x : x;
+            double y = 2.0 * s - 1.0, p[MAXP + 1];
+            p[0] = 1.0;
+            p[1] = y;
+            for (int i = 2; i <= MAXP; ++i)
+                p[i] = ((double)(2 * i - 1) * y * p[i - 1] - (double)(i - 1) * p[i - 2]) / (double)i;
+            for (int i = 2; i <= MAXP; ++i) {
+                tv[t_phi(i, flip) * NPT + q] = (p[i] - p[i - 2]) / (double)(4 * i - 2);
+                td[t_phi(i, flip) * NPT + q] = flip ? -p[i - 1] : p[i - 1];
+            }
+            // scaled Legendre P_i(s; t) with t = s0 + s1 evaluated as the
+            // reference does (HomLegendre of the oriented pair)
+            double t = (1.0 - x) + x;
+            double yy = 2.0 * s - t, tt = t * t, r[MAXP];
+            r[0] = 1.0;
+            r[1] = yy;
+            for (int i = 2; i < MAXP; ++i)
+                r[i] = ((double)(2 * i - 1) * yy * r[i - 1] - (double)(i - 1) * tt * r[i - 2]) / (double)i;
+            for (int i = 0; i < MAXP; ++i) {
+                tv[t_leg(i, flip) * NPT + q] = r[i];
+                td[t_leg(i, flip) * NPT + q] = 0.0;
+            }
+        }
+    }
+}
+
+struct Pt { int l1, l2, l3; };
+__device__ inline Pt point_ix(int l, const int *nq)
+{
+    Pt p;
+    p.l1 = l % nq[0];
+    p.l2 = (l / nq[0]) % nq[1];
+    p.l3 = l / (nq[0] * nq[1]);
+    return p;
+}
+
+// shared state of one element (built by the first wave)
+struct ElemShared {
+    ElemOrders o;
+    int n, nb, nq[3], nint, nh, ok, negjac;
+};
+
+// ------------------------------------------------------------ element kernel
+
+template <typename T> struct Mfma;
+// 16x16x4 MFMA.  Operands: lane l supplies A[l%16][l/16] and B[l/16][l%16].
+// Result register j of lane l holds D[row(l, j)][l%16], with row(l, j) =
+// l/16 + 4j for the f64 instruction and 4(l/16) + j for the f32 one (checked
+// on gfx950 with a single-wave layout probe).
+template <> struct Mfma<double> {
+    typedef d4_t acc_t;
+    __device__ static inline acc_t mma(double a, double b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    __device__ static inline int row(int lane, int j) { return (lane >> 4) + 4 * j; }
+};
+template <> struct Mfma<float> {
+    typedef f4_t acc_t;
+    __device__ static inline acc_t mma(float a, float b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    __device__ static inline int row(int lane, int j) { return 4 * (lane >> 4) + j; }
+};
+
+struct ElemArgs {
+    int space, nel, ngeo, load, ld_k, max_blocks_1d;
+    const int *norder, *nori_e, *nori_f;
+    const double *xnod, *coef;
+    const long long *off_k, *off_v;
+    void *kout;
+    double *bout;
+    int *status;
+};
+
+__device__ inline void getf(int load, const double *x, double *f)
+{
+    const double pi = 3.14159265358979323846;
+    f[0] = f[1] = f[2] = 0.0;
+    switch (load) {
+    case HP3D_LOAD_SINSINSIN: f[0] = 3.0 * pi * pi * sin(pi * x[0]) * sin(pi * x[1]) * sin(pi * x[2]); break;
+    case HP3D_LOAD_CONV: f[0] = -2.0 * sin(pi * x[0]) * sinh(pi * x[1]); break;
+    case HP3D_LOAD_ONE: f[0] = 1.0; break;
+    case HP3D_LOAD_CURL_SIN: f[0] = (1.0 + 2.0 * pi * pi) * sin(pi * x[1]) * sin(pi * x[2]); break;
+    default: break;
+    }
+}
+
+// (I, J), I <= J, of block b of the upper block triangle enumerated column by
+// column: column J holds blocks I = 0..J
+__device__ inline void block_ij(int b, int &I, int &J)
+{
+    J = (int)((sqrt(8.0 * b + 1.0) - 1.0) * 0.5);
+    while ((J + 1) * (J + 2) / 2 <= b) ++J;
+    while (J * (J + 1) / 2 > b) --J;
+    I = b - J * (J + 1) / 2;
+}
+
+template <typename T, int SPACE>
+__global__ __launch_bounds__(WG) void elem_kernel(ElemArgs a)
+{
+    typedef Mfma<T> M;
+    typedef typename M::acc_t acc_t;
+    constexpr int NC = SPACE == 0 ? 3 : 6;   // A columns per point
+    constexpr int QC = KC / NC;              // points per A chunk
+    constexpr int GQ = 64;                   // points per geometry pass
+    constexpr int GS = SPACE == 0 ? 10 : 21; // geometry doubles per point
+
+    __shared__ ElemShared es;
+    __shared__ double tv[3][NTAB * NPT], td[3][NTAB * NPT];
+    __shared__ unsigned dsp[2 * BR];   // descriptors of the block's rows
+    __shared__ unsigned dgeo[MAXN_H1]; // H1 descriptors of the geometry dofs
+    __shared__ double geo[GQ * GS];
+    __shared__ T ac[KC * LDA];
+
+    const int e = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int I, J;
+    block_ij(blockIdx.y, I, J);
+    const bool diag = (I == J);
+
+    if (tid == 0) {
+        ElemShared s;
+        s.ok = decode_orders(a.norder + 19 * (size_t)e, a.nori_e + 12 * (size_t)e,
+                             a.nori_f + 6 * (size_t)e, s.o);
+        s.n = s.nb = s.nint = s.nh = 0;
+        s.nq[0] = s.nq[1] = s.nq[2] = 1;
+        if (s.ok) {
+            dof_counts(SPACE, s.o, s.n, s.nb);
+            int nbh;
+            dof_counts(0, s.o, s.nh, nbh);
+            quad_counts(s.o, s.nq);
+            s.nint = s.nq[0] * s.nq[1] * s.nq[2];
+            if (a.ngeo > s.nh) s.ok = 0;
+        }
+        s.negjac = 0;
+        es = s;
+    }
+    __syncthreads();
+    if (!es.ok) {
+        if (tid == 0 && blockIdx.y == 0 && a.status) a.status[e] = HP3D_E_ORDER;
+        return;
+    }
+    const int n = es.n, nint = es.nint;
+    if (I * BR >= n || J * BR >= n) return; // block beyond this element's size
+
+    // 1D tables: one wave per direction
+    if (wave < 3) fill_tables(es.nq[wave], tv[wave], td[wave]);
+    // descriptors of the block's rows and of the geometry dofs
+    for (int r = tid; r < 2 * BR; r += WG) {
+        int g = (r < BR ? I * BR + r : J * BR + r - BR);
+        dsp[r] = g < n ? (SPACE == 0 ? desc_h1(g, es.o) : desc_hcurl(g, es.o)) : 0u;
+    }
+    for (int k = tid; k < a.ngeo; k += WG) dgeo[k] = desc_h1(k, es.o);
+    __syncthreads();
+
+    const double *xn = a.xnod + 3 * (size_t)a.ngeo * e;
+    const double coef = a.coef[e];
+
+    // tiles of this wave: two tile rows (ti) x eight tile columns (tj)
+    const int tiA = diag ? wave : 2 * wave, tiB = diag ? 7 - wave : 2 * wave + 1;
+    const int nrI = min(BR, n - I * BR), nrJ = min(BR, n - J * BR);
+    const int offJ = diag ? 0 : BR;
+    acc_t acc[16];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) acc[t] = acc_t{0, 0, 0, 0};
+    unsigned valid = 0;
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+        int ti = t < 8 ? tiA : tiB, tj = t & 7;
+        bool v = ti * 16 < nrI && tj * 16 < nrJ && (!diag || tj >= ti);
+        valid |= (v ? 1u : 0u) << t;
+    }
+    const int nrows = diag ? BR : 2 * BR;
+    const bool do_load = diag && a.bout != nullptr;
+    double bsum = 0.0; // load of row I*BR + tid (diagonal blocks, tid < BR)
+
+    for (int g0 = 0; g0 < nint; g0 += GQ) {
+        // geometry of points g0 .. g0+GQ-1 (FP64 always): x, J, det J, J^-1
+        for (int lq = tid; lq < GQ && g0 + lq < nint; lq += WG) {
+            Pt p = point_ix(g0 + lq, es.nq);
+            double x[3] = {0, 0, 0}, jm[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+            for (int k = 0; k < a.ngeo; ++k) {
+                unsigned d = dgeo[k];
+                int i0 = desc_ix(d, 0), i1 = desc_ix(d, 1), i2 = desc_ix(d, 2);
+                double v0 = tv[0][i0 * NPT + p.l1], v1 = tv[1][i1 * NPT + p.l2], v2 = tv[2][i2 * NPT + p.l3];
+                double g0_ = td[0][i0 * NPT + p.l1] * v1 * v2;
+                double g1_ = v0 * td[1][i1 * NPT + p.l2] * v2;
+                double g2_ = v0 * v1 * td[2][i2 * NPT + p.l3];
+                double ph = v0 * v1 * v2;
+                for (int c = 0; c < 3; ++c) {
+                    double xc = xn[3 * k + c];
+                    x[c] += xc * ph;
+                    jm[c][0] += xc * g0_;
+                    jm[c][1] += xc * g1_;
+                    jm[c][2] += xc * g2_;
+                }
+            }
+            // determinant and cofactor inverse (geom.F90:57-113)
+            double det = jm[0][0] * jm[1][1] * jm[2][2] + jm[1][0] * jm[2][1] * jm[0][2] +
+                         jm[2][0] * jm[0][1] * jm[1][2] - jm[2][0] * jm[1][1] * jm[0][2] -
+                         jm[0][0] * jm[2][1] * jm[1][2] - jm[1][0] * jm[0][1] * jm[2][2];
+            if (det < 0.0) es.negjac = 1;
+            double inv[3][3]; // inv[i][j] = d xi_i / d x_j
+            inv[0][0] = (jm[1][1] * jm[2][2] - jm[2][1] * jm[1][2]) / det;
+            inv[1][0] = (-jm[1][0] * jm[2][2] + jm[2][0] * jm[1][2]) / det;
+            inv[2][0] = (jm[1][0] * jm[2][1] - jm[2][0] * jm[1][1]) / det;
+            inv[0][1] = (jm[2][1] * jm[0][2] - jm[0][1] * jm[2][2]) / det;
+            inv[1][1] = (jm[0][0] * jm[2][2] - jm[2][0] * jm[0][2]) / det;
+            inv[2][1] = (-jm[0][0] * jm[2][1] + jm[2][0] * jm[0][1]) / det;
+            inv[0][2] = (jm[0][1] * jm[1][2] - jm[1][1] * jm[0][2]) / det;
+            inv[1][2] = (-jm[0][0] * jm[1][2] + jm[1][0] * jm[0][2]) / det;
+            inv[2][2] = (jm[0][0] * jm[1][1] - jm[1][0] * jm[0][1]) / det;
+            double w = (0.5 * c_gw[es.nq[0] - 1][p.l1]) * (0.5 * c_gw[es.nq[1] - 1][p.l2]) *
+                       (0.5 * c_gw[es.nq[2] - 1][p.l3]);
+            double wd = det * w;
+            double f[3];
+            getf(a.load, x, f);
+            double *gp = geo + lq * GS;
+            if (SPACE == 0) {
+                double s = sqrt(coef * wd);
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j) gp[3 * i + j] = inv[i][j] * s; // [ref][phys]
+                gp[9] = f[0] * wd;
+            } else {
+                double sc = sqrt(wd) / det, sm = sqrt(coef * wd); // mu = 1
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j) {
+                        gp[3 * i + j] = jm[i][j] * sc;      // curl: [phys][ref]
+                        gp[9 + 3 * i + j] = inv[i][j] * sm; // mass: [ref][phys]
+                    }
+                for (int c = 0; c < 3; ++c)
+                    gp[18 + c] = (inv[c][0] * f[0] + inv[c][1] * f[1] + inv[c][2] * f[2]) * wd;
+            }
+        }
+        __syncthreads();
+
+        // load vector rows of a diagonal block
+        if (do_load && tid < BR && I * BR + tid < n) {
+            unsigned d = dsp[tid];
+            int i0 = desc_ix(d, 0), i1 = desc_ix(d, 1), i2 = desc_ix(d, 2);
+            for (int lq = 0; lq < GQ && g0 + lq < nint; ++lq) {
+                Pt p = point_ix(g0 + lq, es.nq);
+                double v = tv[0][i0 * NPT + p.l1] * tv[1][i1 * NPT + p.l2] * tv[2][i2 * NPT + p.l3];
+                if (SPACE == 0) bsum += v * geo[lq * GS + 9];
+                else bsum += (desc_neg(d) ? -v : v) * geo[lq * GS + 18 + desc_comp(d)];
+            }
+        }
+
+        const int gend = min(GQ, nint - g0);
+        for (int l0 = 0; l0 < gend; l0 += QC) {
+            // A chunk: column NC*q + c, row r
+            for (int it = tid; it < nrows * QC; it += WG) {
+                int r = it % nrows, q = it / nrows, lq = l0 + q;
+                int g = (r < BR ? I * BR + r : J * BR + r - BR);
+                double out[NC];
+#pragma unroll
+                for (int c = 0; c < NC; ++c) out[c] = 0.0;
+                if (g < n && lq < gend) {
+                    Pt p = point_ix(g0 + lq, es.nq);
+                    unsigned d = dsp[r];
+                    int i0 = desc_ix(d, 0), i1 = desc_ix(d, 1), i2 = desc_ix(d, 2);
+                    double v0 = tv[0][i0 * NPT + p.l1], v1 = tv[1][i1 * NPT + p.l2], v2 = tv[2][i2 * NPT + p.l3];
+                    double d0 = td[0][i0 * NPT + p.l1], d1 = td[1][i1 * NPT + p.l2], d2 = td[2][i2 * NPT + p.l3];
+                    double gr[3] = {d0 * v1 * v2, v0 * d1 * v2, v0 * v1 * d2};
+                    const double *gp = geo + lq * GS;
+                    if (SPACE == 0) {
+#pragma unroll
+                        for (int c = 0; c < 3; ++c)
+                            out[c] = gr[0] * gp[c] + gr[1] * gp[3 + c] + gr[2] * gp[6 + c];
+                    } else {
+                        int cp = desc_comp(d);
+                        double sg = desc_neg(d) ? -1.0 : 1.0;
+                        double val = sg * v0 * v1 * v2;
+                        // curl of val e_cp in reference coordinates: grad(val) x e_cp
+                        double cu[3];
+                        cu[cp] = 0.0;
+                        cu[(cp + 1) % 3] = sg * gr[(cp + 2) % 3];
+                        cu[(cp + 2) % 3] = -sg * gr[(cp + 1) % 3];
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) {
+                            out[c] = gp[3 * c] * cu[0] + gp[3 * c + 1] * cu[1] + gp[3 * c + 2] * cu[2];
+                            out[3 + c] = val * gp[9 + 3 * cp + c];
+                        }
+                    }
+                }
+#pragma unroll
+                for (int c = 0; c < NC; ++c) ac[(NC * q + c) * LDA + r] = (T)out[c];
+            }
+            __syncthreads();
+            // MFMA over the chunk's KC columns, 4 per step
+#pragma unroll
+            for (int k0 = 0; k0 < KC; k0 += 4) {
+                const T *col = ac + (k0 + (lane >> 4)) * LDA;
+                T aA = col[tiA * 16 + (lane & 15)], aB = col[tiB * 16 + (lane & 15)];
+                T b[8];
+#pragma unroll
+                for (int tj = 0; tj < 8; ++tj) b[tj] = col[offJ + tj * 16 + (lane & 15)];
+#pragma unroll
+                for (int t = 0; t < 16; ++t)
+                    if (valid & (1u << t)) acc[t] = M::mma(t < 8 ? aA : aB, b[t & 7], acc[t]);
+            }
+            __syncthreads();
+        }
+    }
+
+    if (do_load && tid < BR && I * BR + tid < n) a.bout[a.off_v[e] + I * BR + tid] = bsum;
+    if (tid == 0 && blockIdx.y == 0 && a.status) a.status[e] = es.negjac ? HP3D_NEG_JACOBIAN : 0;
+
+    // write-out.  Lane holds D[row(lane,j)][lane%16] of tile (ti,tj), i.e.
+    // K(R + row, C + lane%16).  By symmetry that is K(C + lane%16, R + row):
+    // a coalesced column segment of the mirrored tile.
+    T *kout = (T *)a.kout + a.off_k[e];
+    const size_t ld = a.ld_k > 0 ? (size_t)a.ld_k : (size_t)n;
+    T *stage = ac + wave * (16 * 17);
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+        if (!(valid & (1u << t))) continue;
+        int ti = t < 8 ? tiA : tiB, tj = t & 7;
+        int R = I * BR + ti * 16, C = J * BR + tj * 16;
+        int rr = C + (lane & 15);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            int cc = R + M::row(lane, j);
+            if (rr < n && cc < n) kout[(size_t)cc * ld + rr] = acc[t][j];
+        }
+        if (R != C) {
+            // the tile itself, transposed through this wave's LDS stage
+#pragma unroll
+            for (int j = 0; j < 4; ++j) stage[M::row(lane, j) * 17 + (lane & 15)] = acc[t][j];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            int r2 = R + (lane & 15);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                int c2 = C + 4 * (lane >> 4) + j;
+                T v = stage[(lane & 15) * 17 + 4 * (lane >> 4) + j];
+                if (r2 < n && c2 < n) kout[(size_t)c2 * ld + r2] = v;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+    }
+}
+
+// ------------------------------------------------------------ condensation
+
+struct StcArgs {
+    int space, nel, ld_k, ld_c, nbmax;
+    const int *norder; // order codes, or NULL when nsz gives (n, nb) per element
+    const int *nsz;
+    const long long *off_k, *off_v, *off_c, *off_s;
+    const void *kin;
+    const double *bin;
+    double *aii, *bi, *aschur, *bschur, *ws;
+    int *status;
+};
+
+template <typename T>
+__global__ __launch_bounds__(WG) void stc_kernel(StcArgs a)
+{
+    __shared__ double lds_l[STC_LDS_NB * STC_LDS_NB];
+    __shared__ int s_n, s_nb, s_fail;
+    const int e = blockIdx.x, tid = threadIdx.x;
+    if (tid == 0) {
+        int n = 0, nb = 0;
+        bool ok = true;
+        if (a.nsz) {
+            n = a.nsz[2 * e];
+            nb = a.nsz[2 * e + 1];
+        } else {
+            int zero[18] = {0};
+            ElemOrders o;
+            ok = decode_orders(a.norder + 19 * (size_t)e, zero, zero + 12, o);
+            if (ok) dof_counts(a.space, o, n, nb);
+        }
+        s_n = ok ? n : -1;
+        s_nb = nb;
+        s_fail = 0;
+    }
+    __syncthreads();
+    const int n = s_n, nb = s_nb;
+    if (n < 0) {
+        if (tid == 0 && a.status) a.status[e] = HP3D_E_ORDER;
+        return;
+    }
+    const int ni = n - nb;
+    const T *K = (const T *)a.kin + a.off_k[e];
+    const size_t ldk = a.ld_k > 0 ? (size_t)a.ld_k : (size_t)n;
+    const size_t ldc = a.ld_c > 0 ? (size_t)a.ld_c : (size_t)ni;
+    const double *b = a.bin + a.off_v[e];
+    double *aii = a.aii + a.off_c[e];
+    double *bi = a.bi + a.off_v[e];
+    if (nb == 0) { // nothing to condense: copy through (stc.F90:207)
+        for (int idx = tid; idx < ni * ni; idx += WG) {
+            int i = idx % ni, j = idx / ni;
+            aii[j * ldc + i] = (double)K[j * ldk + i];
+        }
+        for (int i = tid; i < ni; i += WG) bi[i] = b[i];
+        if (tid == 0 && a.status) a.status[e] = 0;
+        return;
+    }
+    if (ni == 0) {
+        if (tid == 0 && a.status) a.status[e] = HP3D_E_NOIFACE;
+        return;
+    }
+    double *L = nb <= STC_LDS_NB ? lds_l : a.ws + (size_t)e * a.nbmax * a.nbmax;
+    const int ldl = nb;
+    // Abb (lower triangle) -> L
+    for (int idx = tid; idx < nb * nb; idx += WG) {
+        int i = idx % nb, j = idx / nb;
+        if (i >= j) L[j * ldl + i] = (double)K[(size_t)(ni + j) * ldk + ni + i];
+    }
+    __syncthreads();
+    // right-looking Cholesky, Abb = L L^T
+    for (int j = 0; j < nb; ++j) {
+        if (tid == 0) {
+            double d = L[j * ldl + j];
+            if (!(d > 0.0)) {
+                s_fail = 1;
+                d = 1.0;
+            }
+            L[j * ldl + j] = sqrt(d);
+        }
+        __syncthreads();
+        double djj = L[j * ldl + j];
+        for (int i = j + 1 + tid; i < nb; i += WG) L[j * ldl + i] /= djj;
+        __syncthreads();
+        int m = nb - j - 1;
+        for (int idx = tid; idx < m * m; idx += WG) {
+            int i = j + 1 + idx % m, k = j + 1 + idx / m;
+            if (i >= k) L[k * ldl + i] -= L[j * ldl + i] * L[j * ldl + k];
+        }
+        __syncthreads();
+    }
+    // X = Abb^-1 [Abi | Bb], one column per thread, into ASchur / BSchur
+    double *X = a.aschur + a.off_s[e];
+    double *Xb = a.bschur + a.off_v[e];
+    for (int c = tid; c <= ni; c += WG) {
+        double *x = c < ni ? X + (size_t)c * nb : Xb;
+        for (int i = 0; i < nb; ++i)
+            x[i] = c < ni ? (double)K[(size_t)c * ldk + ni + i] : b[ni + i];
+        for (int i = 0; i < nb; ++i) { // L y = rhs
+            double s = x[i];
+            for (int k = 0; k < i; ++k) s -= L[k * ldl + i] * x[k];
+            x[i] = s / L[i * ldl + i];
+        }
+        for (int i = nb - 1; i >= 0; --i) { // L^T x = y
+            double s = x[i];
+            for (int k = i + 1; k < nb; ++k) s -= L[i * ldl + k] * x[k];
+            x[i] = s / L[i * ldl + i];
+        }
+    }
+    __syncthreads();
+    // Aii - Aib X and Bi - Aib Xb, with Aib(i,b) = K(i, ni+b) = K(ni+b, i)
+    for (int idx = tid; idx < ni * (ni + 1); idx += WG) {
+        int i = idx % ni, j = idx / ni;
+        const double *x = j < ni ? X + (size_t)j * nb : Xb;
+        double s = 0.0;
+        for (int q = 0; q < nb; ++q) s += (double)K[(size_t)i * ldk + ni + q] * x[q];
+        if (j < ni) aii[j * ldc + i] = (double)K[(size_t)j * ldk + i] - s;
+        else bi[i] = b[i] - s;
+    }
+    if (tid == 0 && a.status) a.status[e] = s_fail ? HP3D_E_FACTOR : 0;
+}
+
+struct BwdArgs {
+    int space, nel;
+    const int *norder;
+    const long long *off_s, *off_v;
+    const double *aschur, *bschur, *xi;
+    double *xb;
+};
+
+__global__ __launch_bounds__(WG) void stc_bwd_kernel(BwdArgs a)
+{
+    const int e = blockIdx.x;
+    const int *no = a.norder + 19 * (size_t)e;
+    int zero[18] = {0};
+    ElemOrders o;
+    if (!decode_orders(no, zero, zero + 12, o)) return;
+    int n, nb;
+    dof_counts(a.space, o, n, nb);
+    int ni = n - nb;
+    const double *A = a.aschur + a.off_s[e];
+    const double *xi = a.xi + a.off_v[e];
+    for (int q = threadIdx.x; q < nb; q += WG) {
+        double s = 0.0;
+        for (int i = 0; i < ni; ++i) s += A[(size_t)i * nb + q] * xi[i];
+        a.xb[a.off_v[e] + q] = a.bschur[a.off_v[e] + q] - s;
+    }
+}
+
+// ------------------------------------------------------------ host state
+
+struct Lib {
+    std::mutex mu;
+    int device = -1;
+    hipStream_t stream = nullptr;
+    int max_ndof = 0;
+    // per-element staging (device)
+    int *d_codes = nullptr; // norder[19], nori_e[12], nori_f[6]
+    double *d_xnod = nullptr, *d_coef = nullptr, *d_k = nullptr, *d_b = nullptr;
+    double *d_aii = nullptr, *d_bi = nullptr, *d_as = nullptr, *d_bs = nullptr, *d_ws = nullptr;
+    long long *d_off = nullptr; // zeros
+    int *d_status = nullptr;
+    // pinned host staging
+    double *h_k = nullptr, *h_b = nullptr, *h_as = nullptr, *h_bs = nullptr, *h_x = nullptr;
+    int *h_codes = nullptr, *h_status = nullptr;
+    char err[256] = {0};
+};
+static Lib g;
+
+static int hip_fail(hipError_t r, const char *what)
+{
+    snprintf(g.err, sizeof(g.err), "%s: %s", what, hipGetErrorString(r));
+    return HP3D_E_HIP;
+}
+#define HIPCHK(x)                                     \
+    do {                                              \
+        hipError_t _r = (x);                          \
+        if (_r != hipSuccess) return hip_fail(_r, #x); \
+    } while (0)
+
+static int nblocks_for(int max_ndof)
+{
+    int t = (max_ndof + BR - 1) / BR;
+    return t * (t + 1) / 2;
+}
+
+static int launch_elem(int space, int precision, const ElemArgs &args, int max_ndof,
+                       hipStream_t st)
+{
+    dim3 grid(args.nel, nblocks_for(max_ndof)), block(WG);
+    if (space == HP3D_H1) {
+        if (precision == HP3D_FP64) hipLaunchKernelGGL((elem_kernel<double, 0>), grid, block, 0, st, args);
+        else hipLaunchKernelGGL((elem_kernel<float, 0>), grid, block, 0, st, args);
+    } else {
+        if (precision == HP3D_FP64) hipLaunchKernelGGL((elem_kernel<double, 1>), grid, block, 0, st, args);
+        else hipLaunchKernelGGL((elem_kernel<float, 1>), grid, block, 0, st, args);
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+static int max_bubbles(int space, int max_ndof)
+{
+    // the bubble block is bounded by the interior space of order MAXP but also
+    // by the element size
+    int nb = space == HP3D_H1 ? (MAXP - 1) * (MAXP - 1) * (MAXP - 1) : 3 * MAXP * (MAXP - 1) * (MAXP - 1);
+    return nb < max_ndof ? nb : max_ndof;
+}
+
+} // namespace hp3d
+
+using namespace hp3d;
+
+// ================================================================== C ABI
+
+extern "C" {
+
+const char *hp3d_gpu_last_error(void) { return g.err; }
+
+int hp3d_gpu_ndof(int space, const int norder[19], int *nrdof, int *nrdofb)
+{
+    if (!norder || (space != HP3D_H1 && space != HP3D_HCURL)) return HP3D_E_ARG;
+    int zero[18] = {0};
+    ElemOrders o;
+    if (!decode_orders(norder, zero, zero + 12, o)) return HP3D_E_ORDER;
+    int n, nb;
+    dof_counts(space, o, n, nb);
+    if (nrdof) *nrdof = n;
+    if (nrdofb) *nrdofb = nb;
+    return 0;
+}
+
+int hp3d_gpu_init(int device, int max_ndof)
+{
+    std::lock_guard<std::mutex> lk(g.mu);
+    if (g.device >= 0) return 0;
+    int cnt = 0;
+    if (hipGetDeviceCount(&cnt) != hipSuccess || cnt <= device || device < 0) {
+        snprintf(g.err, sizeof(g.err), "no HIP device %d", device);
+        return HP3D_E_INIT;
+    }
+    HIPCHK(hipSetDevice(device));
+    if (max_ndof <= 0) max_ndof = MAXN_E;
+    g.max_ndof = max_ndof;
+    HIPCHK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+    size_t nn = (size_t)max_ndof * max_ndof;
+    HIPCHK(hipMalloc(&g.d_codes, 64 * sizeof(int)));
+    HIPCHK(hipMalloc(&g.d_xnod, 3 * (size_t)MAXN_H1 * sizeof(double)));
+    HIPCHK(hipMalloc(&g.d_coef, sizeof(double)));
+    HIPCHK(hipMalloc(&g.d_k, nn * sizeof(double)));
+    HIPCHK(hipMalloc(&g.d_b, max_ndof * sizeof(double)));
+    HIPCHK(hipMalloc(&g.d_aii, nn * sizeof(double)));
+    HIPCHK(hipMalloc(&g.d_bi, max_ndof * sizeof(double)));
+    HIPCHK(hipMalloc(&g.d_as, nn * sizeof(double)));
+    HIPCHK(hipMalloc(&g.d_bs, max_ndof * sizeof(double)));
+    int nbm = max_bubbles(HP3D_HCURL, max_ndof);
+    HIPCHK(hipMalloc(&g.d_ws, (size_t)nbm * nbm * sizeof(double)));
+    HIPCHK(hipMalloc(&g.d_off, sizeof(long long)));
+    HIPCHK(hipMemset(g.d_off, 0, sizeof(long long)));
+    HIPCHK(hipMalloc(&g.d_status, 2 * sizeof(int)));
+    HIPCHK(hipHostMalloc(&g.h_k, nn * sizeof(double)));
+    HIPCHK(hipHostMalloc(&g.h_b, max_ndof * sizeof(double)));
+    HIPCHK(hipHostMalloc(&g.h_as, nn * sizeof(double)));
+    HIPCHK(hipHostMalloc(&g.h_bs, max_ndof * sizeof(double)));
+    HIPCHK(hipHostMalloc(&g.h_x, (3 * (size_t)MAXN_H1 + 1) * sizeof(double)));
+    HIPCHK(hipHostMalloc(&g.h_codes, 64 * sizeof(int)));
+    HIPCHK(hipHostMalloc(&g.h_status, 2 * sizeof(int)));
+    g.device = device;
+    return 0;
+}
+
+int hp3d_gpu_finalize(void)
+{
+    std::lock_guard<std::mutex> lk(g.mu);
+    if (g.device < 0) return 0;
+    hipStreamSynchronize(g.stream);
+    void *dev[] = {g.d_codes, g.d_xnod, g.d_coef, g.d_k, g.d_b, g.d_aii, g.d_bi,
+                   g.d_as,    g.d_bs,   g.d_ws,   g.d_off, g.d_status};
+    for (void *p : dev) hipFree(p);
+    void *host[] = {g.h_k, g.h_b, g.h_as, g.h_bs, g.h_x, g.h_codes, g.h_status};
+    for (void *p : host) hipHostFree(p);
+    hipStreamDestroy(g.stream);
+    g.device = -1;
+    return 0;
+}
+
+int hp3d_gpu_elem_batch(int space, int precision, int nel, const int *norder, const int *nori_e,
+                        const int *nori_f, const double *xnod, int ngeo, const double *coef,
+                        int load, const long long *off_k, int ld_k, void *kout,
+                        const long long *off_v, double *bout, int *status, int max_ndof,
+                        void *stream)
+{
+    if ((space != HP3D_H1 && space != HP3D_HCURL) || (precision != HP3D_FP64 && precision != HP3D_FP32))
+        return HP3D_E_ARG;
+    if (nel < 0 || ngeo < 1 || ngeo > MAXN_H1 || max_ndof < 1 || ld_k < 0) return HP3D_E_ARG;
+    if (nel == 0) return 0;
+    if (!norder || !nori_e || !nori_f || !xnod || !coef || !off_k || !kout) return HP3D_E_ARG;
+    if (bout && !off_v) return HP3D_E_ARG;
+    if (ld_k > 0 && ld_k < max_ndof) return HP3D_E_ARG;
+    ElemArgs a;
+    a.space = space;
+    a.nel = nel;
+    a.ngeo = ngeo;
+    a.load = load;
+    a.ld_k = ld_k;
+    a.max_blocks_1d = (max_ndof + BR - 1) / BR;
+    a.norder = norder;
+    a.nori_e = nori_e;
+    a.nori_f = nori_f;
+    a.xnod = xnod;
+    a.coef = coef;
+    a.off_k = off_k;
+    a.off_v = off_v;
+    a.kout = kout;
+    a.bout = bout;
+    a.status = status;
+    return launch_elem(space, precision, a, max_ndof, (hipStream_t)stream);
+}
+
+size_t hp3d_gpu_stc_workspace(int space, int nel, int max_ndof)
+{
+    int nbm = max_bubbles(space, max_ndof);
+    if (nbm <= STC_LDS_NB) return 0;
+    return (size_t)nel * nbm * nbm * sizeof(double);
+}
+
+int hp3d_gpu_stc_batch(int space, int precision, int nel, const int *norder,
+                       const long long *off_k, int ld_k, const void *kin,
+                       const long long *off_v, const double *bin, const long long *off_c,
+                       int ld_c, double *aii, double *bi, const long long *off_s,
+                       double *aschur, double *bschur, void *workspace, int *status,
+                       int max_ndof, void *stream)
+{
+    if ((space != HP3D_H1 && space != HP3D_HCURL) || (precision != HP3D_FP64 && precision != HP3D_FP32))
+        return HP3D_E_ARG;
+    if (nel < 0 || max_ndof < 1 || ld_k < 0 || ld_c < 0) return HP3D_E_ARG;
+    if (nel == 0) return 0;
+    if (!norder || !off_k || !kin || !off_v || !bin || !off_c || !aii || !bi || !off_s ||
+        !aschur || !bschur)
+        return HP3D_E_ARG;
+    if (hp3d_gpu_stc_workspace(space, nel, max_ndof) > 0 && !workspace) return HP3D_E_ARG;
+    if (precision == HP3D_FP32 && (const void *)aii == kin) return HP3D_E_ARG;
+    StcArgs a;
+    a.space = space;
+    a.nel = nel;
+    a.ld_k = ld_k;
+    a.ld_c = ld_c;
+    a.nbmax = max_bubbles(space, max_ndof);
+    a.norder = norder;
+    a.nsz = nullptr;
+    a.off_k = off_k;
+    a.off_v = off_v;
+    a.off_c = off_c;
+    a.off_s = off_s;
+    a.kin = kin;
+    a.bin = bin;
+    a.aii = aii;
+    a.bi = bi;
+    a.aschur = aschur;
+    a.bschur = bschur;
+    a.ws = (double *)workspace;
+    a.status = status;
+    hipStream_t st = (hipStream_t)stream;
+    if (precision == HP3D_FP64) hipLaunchKernelGGL(stc_kernel<double>, dim3(nel), dim3(WG), 0, st, a);
+    else hipLaunchKernelGGL(stc_kernel<float>, dim3(nel), dim3(WG), 0, st, a);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int hp3d_gpu_stc_bwd_batch(int space, int nel, const int *norder, const long long *off_s,
+                           const double *aschur, const long long *off_v, const double *bschur,
+                           const double *xi, double *xb, void *stream)
+{
+    if ((space != HP3D_H1 && space != HP3D_HCURL) || nel < 0) return HP3D_E_ARG;
+    if (nel == 0) return 0;
+    if (!norder || !off_s || !aschur || !off_v || !bschur || !xi || !xb) return HP3D_E_ARG;
+    BwdArgs a{space, nel, norder, off_s, off_v, aschur, bschur, xi, xb};
+    hipLaunchKernelGGL(stc_bwd_kernel, dim3(nel), dim3(WG), 0, (hipStream_t)stream, a);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// ---------------------------------------------------- per-element drop-in
+
+static int elem_one(int space, const int norder[19], const int nori_e[12], const int nori_f[6],
+                    const double *xnod, int ngeo, double coef, int load, int &n, int &nb)
+{
+    if (g.device < 0) {
+        snprintf(g.err, sizeof(g.err), "hp3d_gpu_init not called");
+        return HP3D_E_INIT;
+    }
+    if (!norder || !nori_e || !nori_f || !xnod || ngeo < 1 || ngeo > MAXN_H1) return HP3D_E_ARG;
+    ElemOrders o;
+    if (!decode_orders(norder, nori_e, nori_f, o)) return HP3D_E_ORDER;
+    dof_counts(space, o, n, nb);
+    int nh, nbh;
+    dof_counts(HP3D_H1, o, nh, nbh);
+    if (n > g.max_ndof || ngeo > nh) return HP3D_E_ARG;
+    memcpy(g.h_codes, norder, 19 * sizeof(int));
+    memcpy(g.h_codes + 19, nori_e, 12 * sizeof(int));
+    memcpy(g.h_codes + 31, nori_f, 6 * sizeof(int));
+    memcpy(g.h_x, xnod, 3 * (size_t)ngeo * sizeof(double));
+    g.h_x[3 * ngeo] = coef;
+    HIPCHK(hipMemcpyAsync(g.d_codes, g.h_codes, 37 * sizeof(int), hipMemcpyHostToDevice, g.stream));
+    HIPCHK(hipMemcpyAsync(g.d_xnod, g.h_x, 3 * (size_t)ngeo * sizeof(double), hipMemcpyHostToDevice, g.stream));
+    HIPCHK(hipMemcpyAsync(g.d_coef, g.h_x + 3 * ngeo, sizeof(double), hipMemcpyHostToDevice, g.stream));
+    int r = hp3d_gpu_elem_batch(space, HP3D_FP64, 1, g.d_codes, g.d_codes + 19, g.d_codes + 31,
+                                g.d_xnod, ngeo, g.d_coef, load, g.d_off, 0, g.d_k, g.d_off, g.d_b,
+                                g.d_status, n, g.stream);
+    return r;
+}
+
+static int copy_back_matrix(double *dst, int ld, const double *src_dev, int rows, int cols,
+                            int ld_src)
+{
+    HIPCHK(hipMemcpyAsync(g.h_k, src_dev, (size_t)ld_src * cols * sizeof(double),
+                          hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    for (int j = 0; j < cols; ++j)
+        memcpy(dst + (size_t)j * ld, g.h_k + (size_t)j * ld_src, rows * sizeof(double));
+    return 0;
+}
+
+int hp3d_gpu_elem(int space, const int norder[19], const int nori_e[12], const int nori_f[6],
+                  const double *xnod, int ngeo, double coef, int load, double *aloc,
+                  int ld_aloc, double *bloc)
+{
+    if (space != HP3D_H1 && space != HP3D_HCURL) return HP3D_E_ARG;
+    std::lock_guard<std::mutex> lk(g.mu);
+    int n, nb;
+    int r = elem_one(space, norder, nori_e, nori_f, xnod, ngeo, coef, load, n, nb);
+    if (r) return r;
+    if (!aloc || !bloc || ld_aloc < n) return HP3D_E_ARG;
+    HIPCHK(hipMemcpyAsync(g.h_b, g.d_b, n * sizeof(double), hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipMemcpyAsync(g.h_status, g.d_status, sizeof(int), hipMemcpyDeviceToHost, g.stream));
+    if ((r = copy_back_matrix(aloc, ld_aloc, g.d_k, n, n, n))) return r;
+    memcpy(bloc, g.h_b, n * sizeof(double));
+    return *g.h_status;
+}
+
+static int stc_run(int space, int n, int nb, const int *d_norder, const int *d_nsz,
+                   const double *d_k, const double *d_b, int *d_status)
+{
+    StcArgs a;
+    a.space = space;
+    a.nel = 1;
+    a.ld_k = 0;
+    a.ld_c = 0;
+    a.nbmax = nb;
+    a.norder = d_norder;
+    a.nsz = d_nsz;
+    a.off_k = a.off_v = a.off_c = a.off_s = g.d_off;
+    a.kin = d_k;
+    a.bin = d_b;
+    a.aii = g.d_aii;
+    a.bi = g.d_bi;
+    a.aschur = g.d_as;
+    a.bschur = g.d_bs;
+    a.ws = g.d_ws;
+    a.status = d_status;
+    (void)n;
+    hipLaunchKernelGGL(stc_kernel<double>, dim3(1), dim3(WG), 0, g.stream, a);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// copy the condensed system back into the caller's ALOC/BLOC and CLOC arrays
+static int stc_copy_back(int ni, int nb, double *aloc, int ld_aloc, double *bloc, double *aschur,
+                         double *bschur, int &st)
+{
+    int r;
+    HIPCHK(hipMemcpyAsync(g.h_status + 1, g.d_status + 1, sizeof(int), hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipMemcpyAsync(g.h_b, g.d_bi, ni * sizeof(double), hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipMemcpyAsync(g.h_bs, g.d_bs, nb * sizeof(double), hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipMemcpyAsync(g.h_as, g.d_as, (size_t)nb * ni * sizeof(double), hipMemcpyDeviceToHost, g.stream));
+    if ((r = copy_back_matrix(aloc, ld_aloc, g.d_aii, ni, ni, ni))) return r;
+    memcpy(bloc, g.h_b, ni * sizeof(double));
+    if (aschur) memcpy(aschur, g.h_as, (size_t)nb * ni * sizeof(double));
+    if (bschur) memcpy(bschur, g.h_bs, nb * sizeof(double));
+    st = g.h_status[1];
+    return 0;
+}
+
+int hp3d_gpu_stc_fwd(int ni, int nb, double *aloc, int ld_aloc, double *bloc, double *aschur,
+                     double *bschur)
+{
+    std::lock_guard<std::mutex> lk(g.mu);
+    if (g.device < 0) {
+        snprintf(g.err, sizeof(g.err), "hp3d_gpu_init not called");
+        return HP3D_E_INIT;
+    }
+    int n = ni + nb;
+    if (ni < 0 || nb < 0 || n > g.max_ndof || !aloc || !bloc || ld_aloc < n) return HP3D_E_ARG;
+    if (nb == 0) return 0;
+    if (ni == 0) return HP3D_E_NOIFACE;
+    if (nb > max_bubbles(HP3D_HCURL, g.max_ndof)) return HP3D_E_ARG;
+    // stage ALOC (ld_aloc) and BLOC into packed device buffers
+    for (int j = 0; j < n; ++j) memcpy(g.h_k + (size_t)j * n, aloc + (size_t)j * ld_aloc, n * sizeof(double));
+    memcpy(g.h_b, bloc, n * sizeof(double));
+    g.h_codes[0] = n;
+    g.h_codes[1] = nb;
+    HIPCHK(hipMemcpyAsync(g.d_k, g.h_k, (size_t)n * n * sizeof(double), hipMemcpyHostToDevice, g.stream));
+    HIPCHK(hipMemcpyAsync(g.d_b, g.h_b, n * sizeof(double), hipMemcpyHostToDevice, g.stream));
+    HIPCHK(hipMemcpyAsync(g.d_codes, g.h_codes, 2 * sizeof(int), hipMemcpyHostToDevice, g.stream));
+    int r = stc_run(HP3D_H1, n, nb, nullptr, g.d_codes, g.d_k, g.d_b, g.d_status + 1);
+    if (r) return r;
+    int st;
+    if ((r = stc_copy_back(ni, nb, aloc, ld_aloc, bloc, aschur, bschur, st))) return r;
+    return st;
+}
+
+int hp3d_gpu_celem(int space, const int norder[19], const int nori_e[12], const int nori_f[6],
+                   const double *xnod, int ngeo, double coef, int load, double *aloc,
+                   int ld_aloc, double *bloc, double *aschur, double *bschur)
+{
+    if (space != HP3D_H1 && space != HP3D_HCURL) return HP3D_E_ARG;
+    std::lock_guard<std::mutex> lk(g.mu);
+    int n, nb;
+    int r = elem_one(space, norder, nori_e, nori_f, xnod, ngeo, coef, load, n, nb);
+    if (r) return r;
+    if (!aloc || !bloc || ld_aloc < n) return HP3D_E_ARG;
+    int ni = n - nb;
+    HIPCHK(hipMemcpyAsync(g.h_status, g.d_status, sizeof(int), hipMemcpyDeviceToHost, g.stream));
+    if (nb == 0) {
+        HIPCHK(hipMemcpyAsync(g.h_b, g.d_b, n * sizeof(double), hipMemcpyDeviceToHost, g.stream));
+        if ((r = copy_back_matrix(aloc, ld_aloc, g.d_k, n, n, n))) return r;
+        memcpy(bloc, g.h_b, n * sizeof(double));
+        return g.h_status[0];
+    }
+    if (ni == 0) return HP3D_E_NOIFACE;
+    if ((r = stc_run(space, n, nb, g.d_codes, nullptr, g.d_k, g.d_b, g.d_status + 1))) return r;
+    int st;
+    if ((r = stc_copy_back(ni, nb, aloc, ld_aloc, bloc, aschur, bschur, st))) return r;
+    return st < 0 ? st : g.h_status[0];
+}
+
+} // extern "C"

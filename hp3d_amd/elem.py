@@ -1,0 +1,289 @@
+"""Host-side mirror of hp3D's element interface over libhp3d_gpu.
+
+Per-element drop-ins (host arrays, one C-ABI call each), mirroring
+  elem / elem_opt        problems/POISSON/GALERKIN/elem.F90:20-55, elem_opt.F90:22-141
+                         problems/MAXWELL/GALERKIN/elem_opt.F90:22-158 (real form)
+  stc_fwd_wrapper        src/modules/stc.F90:182-311
+  celem_systemI up to    src/constrs/celem_systemI.F90:523-534 (elem + stc_fwd)
+and the batched form of the mumps_sc Step 2 element loop
+(src/solver/omp_mumps/mumps_sc.F90:224-342) over device tensors.
+
+Matrices are column-major per element, as ALOC is: element e's K occupies
+kflat[off_k[e] : off_k[e] + N_e*N_e] with leading dimension N_e.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import (FP32, FP64, H1, HCURL, LOAD_CURL_SIN, LOAD_ONE, LOAD_SINSINSIN,  # noqa: F401
+                   LOAD_ZERO, LOAD_CONV, Hp3dError, check)
+
+MAXP = 6
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _tptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+# ------------------------------------------------------------------ counts
+
+def decode_orders(norder):
+    """(edge orders (..,12), face h (..,6), face v (..,6), interior (..,3))."""
+    no = np.asarray(norder)
+    pe = no[..., :12]
+    fh = no[..., 12:18] // 10
+    fv = no[..., 12:18] % 10
+    b = np.stack([no[..., 18] // 100, (no[..., 18] // 10) % 10, no[..., 18] % 10], -1)
+    return pe, fh, fv, b
+
+
+def ndof(space, norder):
+    """(N, nb) per element: celndof / ndof_nod (element/util/celndof.F90:24,
+    element_data.F90:845-855) for hexahedra, nb = bubbles of the middle node."""
+    pe, fh, fv, b = decode_orders(norder)
+    if space == H1:
+        nb = (b[..., 0] - 1) * (b[..., 1] - 1) * (b[..., 2] - 1)
+        n = 8 + (pe - 1).sum(-1) + ((fh - 1) * (fv - 1)).sum(-1) + nb
+    else:
+        nb = (b[..., 0] * (b[..., 1] - 1) * (b[..., 2] - 1) + (b[..., 0] - 1) * b[..., 1] *
+              (b[..., 2] - 1) + (b[..., 0] - 1) * (b[..., 1] - 1) * b[..., 2])
+        n = pe.sum(-1) + (fh * (fv - 1) + (fh - 1) * fv).sum(-1) + nb
+    return n, nb
+
+
+def check_orders(norder):
+    """checkorder (element/shape_1/CheckOrder.F90:139-152): orders in [1, MAXP]."""
+    pe, fh, fv, b = decode_orders(norder)
+    allp = np.concatenate([pe.reshape(-1), fh.reshape(-1), fv.reshape(-1), b.reshape(-1)])
+    if allp.size and (allp.min() < 1 or allp.max() > MAXP):
+        raise Hp3dError("order outside [1, %d]" % MAXP)
+
+
+def nquad(norder, nori_f):
+    """Gauss points per element direction (set_3D_int brick branch with
+    find_order_loc, src/element/quadrature/set_3D_int.F90:203-244)."""
+    pe, fh, fv, b = decode_orders(norder)
+    nf = np.atleast_2d(nori_f)
+    pe, fh, fv, b = [np.atleast_2d(x) for x in (pe, fh, fv, b)]
+    m = b.copy()
+    edir = [0, 1, 0, 1, 0, 1, 0, 1, 2, 2, 2, 2]
+    for e in range(12):
+        m[:, edir[e]] = np.maximum(m[:, edir[e]], pe[:, e])
+    axes = [(0, 1), (0, 1), (0, 2), (1, 2), (0, 2), (1, 2)]
+    swapped = np.array([0, 1, 0, 1, 1, 0, 1, 0])[nf]
+    for f in range(6):
+        o0 = np.where(swapped[:, f] == 1, fv[:, f], fh[:, f])
+        o1 = np.where(swapped[:, f] == 1, fh[:, f], fv[:, f])
+        m[:, axes[f][0]] = np.maximum(m[:, axes[f][0]], o0)
+        m[:, axes[f][1]] = np.maximum(m[:, axes[f][1]], o1)
+    return np.minimum(m, MAXP) + 1
+
+
+# ---------------------------------------------------- per-element drop-ins
+
+def _codes(norder, nori_e, nori_f):
+    no = np.ascontiguousarray(np.asarray(norder, dtype=np.int32).reshape(19))
+    oe = np.ascontiguousarray(np.asarray(nori_e, dtype=np.int32).reshape(12))
+    of = np.ascontiguousarray(np.asarray(nori_f, dtype=np.int32).reshape(6))
+    return no, oe, of
+
+
+def elem(space, norder, nori_e, nori_f, xnod, coef=1.0, load=LOAD_SINSINSIN, ld=None):
+    """elem_opt for one element on the GPU.  xnod: (ngeo, 3) geometry dofs.
+
+    Returns (K (N,N), b (N,), flag) with flag 1 when a negative Jacobian was seen."""
+    lib = _lib.init()
+    no, oe, of = _codes(norder, nori_e, nori_f)
+    n, _ = ndof(space, no)
+    n = int(n)
+    ld = n if ld is None else int(ld)
+    x = np.ascontiguousarray(xnod, dtype=np.float64)
+    aloc = np.zeros((ld, n), order="F")  # ALOC(1,1)%array with leading dimension ld
+    bloc = np.zeros(n)
+    rc = lib.hp3d_gpu_elem(space, _ptr(no), _ptr(oe), _ptr(of), _ptr(x), x.shape[0],
+                           float(coef), int(load), _ptr(aloc), ld, _ptr(bloc))
+    flag = check(rc, "hp3d_gpu_elem")
+    return np.array(aloc[:n, :n]), bloc, flag
+
+
+def stc_fwd(k, b, nb):
+    """stc_fwd_wrapper for one element: condense the last nb dofs.
+
+    Returns (Aii (ni,ni), Bi (ni,), ASchur (nb,ni), BSchur (nb,))."""
+    lib = _lib.init()
+    n = k.shape[0]
+    ni = n - nb
+    a = np.asfortranarray(np.array(k, dtype=np.float64))
+    bb = np.array(b, dtype=np.float64)
+    asch = np.zeros((nb, ni), order="F")
+    bsch = np.zeros(nb)
+    rc = lib.hp3d_gpu_stc_fwd(ni, nb, _ptr(a), n, _ptr(bb), _ptr(asch), _ptr(bsch))
+    check(rc, "hp3d_gpu_stc_fwd")
+    return np.array(a[:ni, :ni]), bb[:ni].copy(), np.array(asch), bsch
+
+
+def celem(space, norder, nori_e, nori_f, xnod, coef=1.0, load=LOAD_SINSINSIN):
+    """elem + stc_fwd for one element: (Aii, Bi, ASchur, BSchur, flag)."""
+    lib = _lib.init()
+    no, oe, of = _codes(norder, nori_e, nori_f)
+    n, nb = ndof(space, no)
+    n, nb = int(n), int(nb)
+    ni = n - nb
+    x = np.ascontiguousarray(xnod, dtype=np.float64)
+    a = np.zeros((n, n), order="F")
+    bl = np.zeros(n)
+    asch = np.zeros((max(nb, 1), ni), order="F")
+    bsch = np.zeros(max(nb, 1))
+    rc = lib.hp3d_gpu_celem(space, _ptr(no), _ptr(oe), _ptr(of), _ptr(x), x.shape[0],
+                            float(coef), int(load), _ptr(a), n, _ptr(bl), _ptr(asch), _ptr(bsch))
+    flag = check(rc, "hp3d_gpu_celem")
+    if nb == 0:
+        return np.array(a), bl, np.zeros((0, n)), np.zeros(0), flag
+    return np.array(a[:ni, :ni]), bl[:ni].copy(), np.array(asch), bsch, flag
+
+
+# ------------------------------------------------------------ batched form
+
+class ElementBatch:
+    """Element data resident in device memory, ready for the batched kernels.
+
+    norder (nel,19), nori_e (nel,12), nori_f (nel,6) int; xnod (nel,ngeo,3);
+    coef (nel,).  Outputs are packed: element e's N_e x N_e matrix starts at
+    off_k[e] with leading dimension N_e.
+    """
+
+    def __init__(self, space, norder, nori_e, nori_f, xnod, coef, load=LOAD_SINSINSIN,
+                 device=None):
+        import torch
+
+        _lib.init()
+        self.space = space
+        self.load = int(load)
+        norder = np.ascontiguousarray(norder, dtype=np.int32)
+        check_orders(norder)
+        self.nel = norder.shape[0]
+        self.n, self.nb = [np.asarray(v, dtype=np.int64) for v in ndof(space, norder)]
+        self.ni = self.n - self.nb
+        self.max_ndof = int(self.n.max()) if self.nel else 1
+        self.norder_h = norder
+        dev = torch.device("cuda") if device is None else torch.device(device)
+        self.device = dev
+        t = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt).to(dev)
+        self.norder = t(norder, torch.int32)
+        self.nori_e = t(nori_e, torch.int32)
+        self.nori_f = t(nori_f, torch.int32)
+        xnod = np.ascontiguousarray(xnod, dtype=np.float64)
+        self.ngeo = xnod.shape[1]
+        self.xnod = t(xnod, torch.float64)
+        self.coef = t(coef, torch.float64)
+        # packed offsets
+        n2 = self.n * self.n
+        self.off_k_h = np.concatenate([[0], np.cumsum(n2)[:-1]]).astype(np.int64)
+        self.off_v_h = np.concatenate([[0], np.cumsum(self.n)[:-1]]).astype(np.int64)
+        ni2 = self.ni * self.ni
+        self.off_c_h = np.concatenate([[0], np.cumsum(ni2)[:-1]]).astype(np.int64)
+        nbi = self.nb * self.ni
+        self.off_s_h = np.concatenate([[0], np.cumsum(nbi)[:-1]]).astype(np.int64)
+        self.size_k = int(n2.sum())
+        self.size_v = int(self.n.sum())
+        self.size_c = int(ni2.sum())
+        self.size_s = int(nbi.sum())
+        self.off_k = t(self.off_k_h, torch.int64)
+        self.off_v = t(self.off_v_h, torch.int64)
+        self.off_c = t(self.off_c_h, torch.int64)
+        self.off_s = t(self.off_s_h, torch.int64)
+        self.status = torch.zeros(self.nel, dtype=torch.int32, device=dev)
+        ws = _lib.load().hp3d_gpu_stc_workspace(space, self.nel, self.max_ndof)
+        self.workspace = torch.empty(max(int(ws) // 8, 1), dtype=torch.float64, device=dev)
+
+    @classmethod
+    def from_mesh(cls, space, mesh, load=None, device=None):
+        if load is None:
+            load = LOAD_SINSINSIN if space == H1 else LOAD_CURL_SIN
+        return cls(space, mesh.norder, mesh.nori_e, mesh.nori_f, mesh.xnod(), mesh.coef,
+                   load=load, device=device)
+
+    def alloc_outputs(self, precision=FP64, want_k=True, want_stc=True):
+        import torch
+
+        dt = torch.float64 if precision == FP64 else torch.float32
+        out = {}
+        dev = self.device
+        if want_k:
+            out["k"] = torch.empty(self.size_k, dtype=dt, device=dev)
+            out["b"] = torch.empty(self.size_v, dtype=torch.float64, device=dev)
+        if want_stc:
+            out["aii"] = torch.empty(max(self.size_c, 1), dtype=torch.float64, device=dev)
+            out["bi"] = torch.empty(self.size_v, dtype=torch.float64, device=dev)
+            out["aschur"] = torch.empty(max(self.size_s, 1), dtype=torch.float64, device=dev)
+            out["bschur"] = torch.empty(self.size_v, dtype=torch.float64, device=dev)
+        return out
+
+    def element_matrices(self, out, precision=FP64, stream=None):
+        """Launch the element kernel into out['k'], out['b'] (no sync)."""
+        import torch
+
+        lib = _lib.load()
+        st = torch.cuda.current_stream().cuda_stream if stream is None else stream
+        rc = lib.hp3d_gpu_elem_batch(
+            self.space, precision, self.nel, _tptr(self.norder), _tptr(self.nori_e),
+            _tptr(self.nori_f), _tptr(self.xnod), self.ngeo, _tptr(self.coef), self.load,
+            _tptr(self.off_k), 0, _tptr(out["k"]), _tptr(self.off_v), _tptr(out["b"]),
+            _tptr(self.status), self.max_ndof, ctypes.c_void_p(st))
+        check(rc, "hp3d_gpu_elem_batch")
+
+    def condense(self, out, precision=FP64, stream=None):
+        """Launch static condensation of out['k'], out['b'] (no sync)."""
+        import torch
+
+        lib = _lib.load()
+        st = torch.cuda.current_stream().cuda_stream if stream is None else stream
+        rc = lib.hp3d_gpu_stc_batch(
+            self.space, precision, self.nel, _tptr(self.norder), _tptr(self.off_k), 0,
+            _tptr(out["k"]), _tptr(self.off_v), _tptr(out["b"]), _tptr(self.off_c), 0,
+            _tptr(out["aii"]), _tptr(out["bi"]), _tptr(self.off_s), _tptr(out["aschur"]),
+            _tptr(out["bschur"]), _tptr(self.workspace), _tptr(self.status), self.max_ndof,
+            ctypes.c_void_p(st))
+        check(rc, "hp3d_gpu_stc_batch")
+
+    def run(self, precision=FP64, want_k=True, want_stc=True, out=None):
+        """Element matrices and (optionally) condensation; returns the outputs."""
+        if out is None:
+            out = self.alloc_outputs(precision, True, want_stc)
+        self.element_matrices(out, precision)
+        if want_stc:
+            self.condense(out, precision)
+        return out
+
+    def check_status(self):
+        s = self.status.cpu().numpy()
+        if (s < 0).any():
+            e = int(np.nonzero(s < 0)[0][0])
+            check(int(s[e]), "element %d" % e)
+        return s
+
+    # ---------------------------------------------------------- host views
+    def k_of(self, out, e, key="k"):
+        n = int(self.n[e])
+        o = int(self.off_k_h[e])
+        return out[key][o:o + n * n].double().cpu().numpy().reshape(n, n).T
+
+    def aii_of(self, out, e):
+        ni = int(self.ni[e])
+        o = int(self.off_c_h[e])
+        return out["aii"][o:o + ni * ni].cpu().numpy().reshape(ni, ni).T
+
+    def aschur_of(self, out, e):
+        ni, nb = int(self.ni[e]), int(self.nb[e])
+        o = int(self.off_s_h[e])
+        return out["aschur"][o:o + nb * ni].cpu().numpy().reshape(ni, nb).T
+
+    def vec_of(self, out, key, e, length=None):
+        o = int(self.off_v_h[e])
+        n = int(self.n[e]) if length is None else int(length)
+        return out[key][o:o + n].cpu().numpy()

@@ -1,0 +1,163 @@
+/*
+ * hp3d_gpu.h -- C ABI of the MI355X (gfx950) element-computation library.
+ *
+ * Replaces, for hexahedral H1 and H(curl) elements with trilinear or
+ * isoparametric geometry, the per-element path of hp3D:
+ *   user elem / elem_opt   problems/POISSON/GALERKIN/elem.F90:20-55,
+ *                          problems/POISSON/GALERKIN/elem_opt.F90:22-141,
+ *                          problems/MAXWELL/GALERKIN/elem_opt.F90:22-158
+ *   static condensation    src/modules/stc.F90:182-311 (stc_fwd_wrapper),
+ *                          :338-416 (stc_fwd_herm), :443-509 (stc_fwd_gen)
+ *   back substitution      src/modules/stc.F90:664-679 (stc_bwd)
+ *   the OpenMP element loop src/solver/omp_mumps/mumps_sc.F90:224-342
+ *
+ * Conventions are hp3D's: integer codes as in find_order / find_orient
+ * (norder[19]: 12 edge orders, 6 face orders h*10+v in face-node axes,
+ * interior x*100+y*10+z; nori_e[12] in {0,1}; nori_f[6] in {0..7}),
+ * geometry dofs xnod as 3 x ngeo column-major (ngeo = 8: vertices only,
+ * i.e. a trilinear 'Linear' GMP block; ngeo = N: full isoparametric),
+ * matrices column-major with an explicit leading dimension, element-local
+ * dof order vertices, edges, faces, bubbles (interface first, bubbles last).
+ *
+ * Return codes (all entry points): 0 ok; 1 a negative Jacobian was seen
+ * (geom.F90:70-71 sets Iflag and the reference continues); negative values
+ * are fatal conditions on which the reference stops:
+ *   HP3D_E_ORDER   order outside [1, MAXP]        (CheckOrder.F90:139-152)
+ *   HP3D_E_FACTOR  bubble block not SPD           (stc.F90:360-396, 464-489)
+ *   HP3D_E_NOIFACE no interface dof               (stc.F90:214-217)
+ *   HP3D_E_ARG     bad argument (sizes, pointers, leading dimensions)
+ *   HP3D_E_HIP     HIP runtime error
+ *   HP3D_E_INIT    library not initialised / no device
+ */
+#ifndef HP3D_GPU_H
+#define HP3D_GPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HP3D_MAXP 6
+
+#define HP3D_OK 0
+#define HP3D_NEG_JACOBIAN 1
+#define HP3D_E_ORDER (-1)
+#define HP3D_E_FACTOR (-4)
+#define HP3D_E_NOIFACE (-3)
+#define HP3D_E_ARG (-5)
+#define HP3D_E_HIP (-6)
+#define HP3D_E_INIT (-7)
+
+/* energy spaces */
+#define HP3D_H1 0    /* Poisson: (kappa grad u, grad v) */
+#define HP3D_HCURL 1 /* (mu^-1 curl E, curl F) + (eps E, F), real form */
+
+/* arithmetic of the B^T W B contraction; geometry, Jacobian inverse and the
+ * bubble factorisation are always FP64 */
+#define HP3D_FP64 0
+#define HP3D_FP32 1
+
+/* load selectors (the user getf routine) */
+#define HP3D_LOAD_ZERO 0
+#define HP3D_LOAD_SINSINSIN 1 /* f = 3 pi^2 sin(pi x) sin(pi y) sin(pi z)     */
+#define HP3D_LOAD_CONV 2      /* f = -2 sin(pi x) sinh(pi y)                 */
+#define HP3D_LOAD_ONE 3       /* f = 1                                        */
+#define HP3D_LOAD_CURL_SIN 10 /* J = (1 + 2 pi^2) sin(pi y) sin(pi z) e_x     */
+
+/* ------------------------------------------------------------ lifecycle */
+
+/* Select the device and create the library's stream and pinned staging for
+ * the per-element entries.  No allocation happens in later per-element calls
+ * unless an element exceeds the staging sized here (max_ndof dofs, 0 = MAXP
+ * maximum).  Replaces the THREADPRIVATE workspace setup of mumps_sc Step 2
+ * (omp_mumps/mumps_sc.F90:228-251). */
+int hp3d_gpu_init(int device, int max_ndof);
+int hp3d_gpu_finalize(void);
+/* last HIP error string (for messages) */
+const char *hp3d_gpu_last_error(void);
+
+/* ------------------------------------------------------------- counts */
+
+/* celndof / ndof_nod for a hexahedron (element/util/celndof.F90:24,
+ * element_data.F90:808-883): total dofs, and bubble dofs of the middle node
+ * (stc_get_nrdof, stc.F90:94-160). */
+int hp3d_gpu_ndof(int space, const int norder[19], int *nrdof, int *nrdofb);
+
+/* ------------------------------------------------ per-element drop-in */
+
+/* elem_opt for one element, host pointers.  Writes the N x N stiffness into
+ * aloc (leading dimension ld_aloc >= N, the caller's ALOC(1,1)%array) and the
+ * load into bloc.  coef is kappa_e (H1) or eps_e (H(curl), mu = 1).
+ * Replaces POISSON/GALERKIN/elem_opt.F90:22-141 and the real form of
+ * MAXWELL/GALERKIN/elem_opt.F90:22-158. */
+int hp3d_gpu_elem(int space, const int norder[19], const int nori_e[12], const int nori_f[6],
+                  const double *xnod, int ngeo, double coef, int load, double *aloc,
+                  int ld_aloc, double *bloc);
+
+/* stc_fwd_wrapper for one element, host pointers: condenses the last nb of
+ * ni+nb dofs of aloc/bloc in place (aloc(1:ni,1:ni) <- Aii - Aib Abb^-1 Abi,
+ * bloc(1:ni) <- Bi - Aib Abb^-1 Bb) and returns ASchur = Abb^-1 Abi (nb x ni,
+ * leading dimension nb) and BSchur = Abb^-1 Bb (either may be NULL).
+ * Replaces stc.F90:182-311 with stc_fwd_herm semantics (Cholesky; Abb is SPD
+ * for these forms, where stc_fwd_gen's LU gives the same factors' solution). */
+int hp3d_gpu_stc_fwd(int ni, int nb, double *aloc, int ld_aloc, double *bloc, double *aschur,
+                     double *bschur);
+
+/* elem + stc_fwd_wrapper fused for one element (celem_systemI.F90:523-534). */
+int hp3d_gpu_celem(int space, const int norder[19], const int nori_e[12], const int nori_f[6],
+                   const double *xnod, int ngeo, double coef, int load, double *aloc,
+                   int ld_aloc, double *bloc, double *aschur, double *bschur);
+
+/* ----------------------------------------------- batched device entries */
+
+/*
+ * Element matrices and loads for nel elements, all pointers DEVICE pointers,
+ * launched on `stream` (a hipStream_t, NULL = default stream).  Per element e:
+ *   norder[19e..], nori_e[12e..], nori_f[6e..], xnod[3*ngeo*e..], coef[e]
+ *   K at kout + off_k[e] (FP64 or FP32 by `precision`), leading dimension
+ *     ld_k, or N_e when ld_k == 0 (packed)
+ *   load at bout + off_v[e] (FP64; bout may be NULL)
+ *   status[e] (may be NULL): 0, 1 (negative Jacobian) or HP3D_E_ORDER
+ * max_ndof bounds N_e over the batch (it sizes the launch grid).
+ * Replaces the element loop of mumps_sc Step 2 up to elem
+ * (mumps_sc.F90:253-259 -> celem_systemI.F90:523).
+ */
+int hp3d_gpu_elem_batch(int space, int precision, int nel, const int *norder, const int *nori_e,
+                        const int *nori_f, const double *xnod, int ngeo, const double *coef,
+                        int load, const long long *off_k, int ld_k, void *kout,
+                        const long long *off_v, double *bout, int *status, int max_ndof,
+                        void *stream);
+
+/*
+ * Static condensation of nel element systems already in device memory
+ * (stc_fwd_wrapper + stc_fwd_herm, stc.F90:182-416), FP64 arithmetic:
+ *   kin  at off_k[e], ld_k (0 = N_e), FP64 or FP32 by `precision`
+ *   bin  at off_v[e]
+ *   aii  at off_c[e], ld_c (0 = ni_e): condensed matrix (may alias kin when
+ *        precision is FP64, off_c == off_k and ld_c == ld_k: in place as ALOC)
+ *   bi   at off_v[e]: condensed load (may alias bin)
+ *   aschur at off_s[e] (nb_e x ni_e, leading dimension nb_e), bschur at off_v[e]
+ *   workspace: device memory of hp3d_gpu_stc_workspace(...) bytes
+ */
+int hp3d_gpu_stc_batch(int space, int precision, int nel, const int *norder,
+                       const long long *off_k, int ld_k, const void *kin,
+                       const long long *off_v, const double *bin, const long long *off_c,
+                       int ld_c, double *aii, double *bi, const long long *off_s,
+                       double *aschur, double *bschur, void *workspace, int *status,
+                       int max_ndof, void *stream);
+size_t hp3d_gpu_stc_workspace(int space, int nel, int max_ndof);
+
+/*
+ * Back substitution for nel elements (stc_bwd, stc.F90:664-679):
+ * xb = bschur - aschur * xi, with xi at off_v[e] in `xi` and xb at off_v[e]
+ * in `xb` (device pointers).
+ */
+int hp3d_gpu_stc_bwd_batch(int space, int nel, const int *norder, const long long *off_s,
+                           const double *aschur, const long long *off_v, const double *bschur,
+                           const double *xi, double *xb, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

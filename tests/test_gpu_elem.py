@@ -1,0 +1,116 @@
+"""Parity of the HIP element path (through the C ABI) against the oracle.
+
+Tolerances: FP64 element, load and condensed matrices 1e-12 relative
+Frobenius per element; FP32 contraction 1e-6 relative Frobenius for K.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from hp3d_amd import mesh
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+TOL32 = 1e-6
+
+
+def rel(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def run_batch(m, space, precision, want_stc=True, herm=True):
+    import torch
+
+    import hp3d_amd
+
+    batch = hp3d_amd.ElementBatch.from_mesh(space, m)
+    out = batch.run(precision, want_stc=want_stc)
+    torch.cuda.synchronize()
+    batch.check_status()
+    load = oracle.LOAD_SINSINSIN if space == 0 else oracle.LOAD_CURL_SIN
+    ldk = int(batch.n.max())
+    ref = oracle.element_loop(space, m.norder, m.nori_e, m.nori_f, m.xnod(), m.coef, load,
+                              ldk=ldk, want_stc=want_stc, herm=herm)
+    return batch, out, ref
+
+
+def compare(batch, out, ref, tol_k, tol_c, want_stc=True, key="k"):
+    worst = {"k": 0.0, "b": 0.0, "aii": 0.0, "bi": 0.0, "aschur": 0.0, "bschur": 0.0}
+    for e in range(batch.nel):
+        n, nb, ni = int(batch.n[e]), int(batch.nb[e]), int(batch.ni[e])
+        worst["k"] = max(worst["k"], rel(batch.k_of(out, e), ref["k"][e].T[:n, :n]))
+        worst["b"] = max(worst["b"], rel(batch.vec_of(out, "b", e), ref["b"][e][:n]))
+        if want_stc and nb > 0:
+            worst["aii"] = max(worst["aii"], rel(batch.aii_of(out, e), ref["aii"][e].T[:ni, :ni]))
+            worst["bi"] = max(worst["bi"], rel(batch.vec_of(out, "bi", e, ni), ref["bi"][e][:ni]))
+            asr = ref["aschur"][e].reshape(-1)[: nb * ni].reshape(ni, nb).T
+            worst["aschur"] = max(worst["aschur"], rel(batch.aschur_of(out, e), asr))
+            worst["bschur"] = max(worst["bschur"],
+                                  rel(batch.vec_of(out, "bschur", e, nb), ref["bschur"][e][:nb]))
+    print({k: "%.2e" % v for k, v in worst.items()})
+    assert worst["k"] < tol_k and worst["b"] < TOL64
+    if want_stc:
+        for k in ("aii", "bi", "aschur", "bschur"):
+            assert worst[k] < tol_c, (k, worst[k])
+    return worst
+
+
+@pytest.mark.parametrize("p", [1, 2, 3, 4])
+def test_h1_uniform_fp64(gpu, p):
+    m = mesh.structured((3, 2, 2), porder=p, perturb=0.15, mesh_seed=p, perm_seed=10 + p,
+                        coef=np.exp(np.random.default_rng(p).normal(size=12)))
+    batch, out, ref = run_batch(m, 0, 0)
+    compare(batch, out, ref, TOL64, TOL64)
+
+
+def test_h1_mixed_orders_fp64(gpu):
+    rng = np.random.default_rng(5)
+    porder = rng.integers(1, 7, size=(4 * 3 * 3, 3))
+    m = mesh.structured((4, 3, 3), porder=porder, perturb=0.1, mesh_seed=3, perm_seed=4,
+                        coef=np.exp(rng.normal(size=36)))
+    batch, out, ref = run_batch(m, 0, 0)
+    compare(batch, out, ref, TOL64, TOL64)
+
+
+def test_h1_large_p6_fp64(gpu):
+    # p = 6: N = 343 > 128, several K blocks per element
+    m = mesh.structured((2, 1, 1), porder=6, perturb=0.1, mesh_seed=1, perm_seed=2)
+    batch, out, ref = run_batch(m, 0, 0)
+    compare(batch, out, ref, TOL64, 1e-11)
+
+
+def test_h1_fp32(gpu):
+    m = mesh.structured((3, 3, 2), porder=4, perturb=0.15, mesh_seed=9, perm_seed=9,
+                        coef=np.exp(np.random.default_rng(9).normal(size=18)))
+    batch, out, ref = run_batch(m, 0, 1)
+    w = compare(batch, out, ref, TOL32, 1e-3)
+    print("fp32 worst", w)
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_hcurl_fp64(gpu, p):
+    m = mesh.structured((2, 2, 1), porder=p, perturb=0.1, mesh_seed=20 + p, perm_seed=30 + p,
+                        coef=np.random.default_rng(p).uniform(1, 4, 4))
+    batch, out, ref = run_batch(m, 1, 0)
+    compare(batch, out, ref, TOL64, 1e-11)
+
+
+def test_dropin_elem_and_stc(gpu):
+    import hp3d_amd
+
+    m = mesh.structured((2, 2, 2), porder=3, perturb=0.1, mesh_seed=1, perm_seed=3)
+    for e in range(m.nel):
+        args = (m.norder[e], m.nori_e[e], m.nori_f[e])
+        k, b, flag = hp3d_amd.elem(0, *args, m.xnod()[e], coef=1.0)
+        kr, br, _ = oracle.elem_h1(*args, m.xnod()[e].T, kappa=1.0)
+        assert flag == 0
+        assert rel(k, kr) < TOL64 and rel(b, br) < TOL64
+        nb = oracle.nbubble_h1(m.norder[e][18])
+        aii, bi, asc, bsc = hp3d_amd.stc_fwd(kr, br, nb)
+        ar, bir, ascr, bscr = oracle.stc_fwd(kr, br, nb, herm=False)
+        assert rel(aii, ar) < TOL64 and rel(bi, bir) < TOL64
+        assert rel(asc, ascr) < TOL64 and rel(bsc, bscr) < TOL64
+        aii2, bi2, asc2, bsc2, fl2 = hp3d_amd.celem(0, *args, m.xnod()[e], coef=1.0)
+        assert rel(aii2, ar) < TOL64 and rel(asc2, ascr) < TOL64
