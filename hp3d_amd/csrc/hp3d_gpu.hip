@@ -158,6 +158,173 @@ __device__ inline void block_ij(int b, int &I, int &J)
     I = b - J * (J + 1) / 2;
 }
 
+// geometry of one quadrature point (FP64): x, J, det J, J^-1 folded with the
+// weights into GS doubles at gp (geom3D.F90:77-91, geom.F90:57-113,
+// elem_opt.F90:101-122).  Returns true when det J < 0.
+template <int SPACE>
+__device__ bool point_geometry(const ElemShared &es, int l, const double (*tv)[NTAB * NPT],
+                               const double (*td)[NTAB * NPT], const unsigned *dgeo,
+                               const double *xn, int ngeo, double coef, int load, double *gp)
+{
+    Pt p = point_ix(l, es.nq);
+    double x[3] = {0, 0, 0}, jm[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int k = 0; k < ngeo; ++k) {
+        unsigned d = dgeo[k];
+        int i0 = desc_ix(d, 0), i1 = desc_ix(d, 1), i2 = desc_ix(d, 2);
+        double v0 = tv[0][i0 * NPT + p.l1], v1 = tv[1][i1 * NPT + p.l2], v2 = tv[2][i2 * NPT + p.l3];
+        double g0 = td[0][i0 * NPT + p.l1] * v1 * v2;
+        double g1 = v0 * td[1][i1 * NPT + p.l2] * v2;
+        double g2 = v0 * v1 * td[2][i2 * NPT + p.l3];
+        double ph = v0 * v1 * v2;
+        for (int c = 0; c < 3; ++c) {
+            double xc = xn[3 * k + c];
+            x[c] += xc * ph;
+            jm[c][0] += xc * g0;
+            jm[c][1] += xc * g1;
+            jm[c][2] += xc * g2;
+        }
+    }
+    double det = jm[0][0] * jm[1][1] * jm[2][2] + jm[1][0] * jm[2][1] * jm[0][2] +
+                 jm[2][0] * jm[0][1] * jm[1][2] - jm[2][0] * jm[1][1] * jm[0][2] -
+                 jm[0][0] * jm[2][1] * jm[1][2] - jm[1][0] * jm[0][1] * jm[2][2];
+    double inv[3][3]; // inv[i][j] = d xi_i / d x_j
+    inv[0][0] = (jm[1][1] * jm[2][2] - jm[2][1] * jm[1][2]) / det;
+    inv[1][0] = (-jm[1][0] * jm[2][2] + jm[2][0] * jm[1][2]) / det;
+    inv[2][0] = (jm[1][0] * jm[2][1] - jm[2][0] * jm[1][1]) / det;
+    inv[0][1] = (jm[2][1] * jm[0][2] - jm[0][1] * jm[2][2]) / det;
+    inv[1][1] = (jm[0][0] * jm[2][2] - jm[2][0] * jm[0][2]) / det;
+    inv[2][1] = (-jm[0][0] * jm[2][1] + jm[2][0] * jm[0][1]) / det;
+    inv[0][2] = (jm[0][1] * jm[1][2] - jm[1][1] * jm[0][2]) / det;
+    inv[1][2] = (-jm[0][0] * jm[1][2] + jm[1][0] * jm[0][2]) / det;
+    inv[2][2] = (jm[0][0] * jm[1][1] - jm[1][0] * jm[0][1]) / det;
+    double w = (0.5 * c_gw[es.nq[0] - 1][p.l1]) * (0.5 * c_gw[es.nq[1] - 1][p.l2]) *
+               (0.5 * c_gw[es.nq[2] - 1][p.l3]);
+    double wd = det * w;
+    double f[3];
+    getf(load, x, f);
+    if (SPACE == 0) {
+        double s = sqrt(coef * wd);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) gp[3 * i + j] = inv[i][j] * s; // [ref][phys]
+        gp[9] = f[0] * wd;
+    } else {
+        double sc = sqrt(wd) / det, sm = sqrt(coef * wd); // mu = 1
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                gp[3 * i + j] = jm[i][j] * sc;      // curl: [phys][ref]
+                gp[9 + 3 * i + j] = inv[i][j] * sm; // mass: [ref][phys]
+            }
+        for (int c = 0; c < 3; ++c)
+            gp[18 + c] = (inv[c][0] * f[0] + inv[c][1] * f[1] + inv[c][2] * f[2]) * wd;
+    }
+    return det < 0.0;
+}
+
+// the NC entries of row d (descriptor) at point p of the A matrix
+// (A_TEST of elem_opt.F90:122, or the [curl | mass] pair of the H(curl) form)
+template <int SPACE>
+__device__ inline void a_entries(unsigned d, Pt p, const double (*tv)[NTAB * NPT],
+                                 const double (*td)[NTAB * NPT], const double *gp, double *out)
+{
+    int i0 = desc_ix(d, 0), i1 = desc_ix(d, 1), i2 = desc_ix(d, 2);
+    double v0 = tv[0][i0 * NPT + p.l1], v1 = tv[1][i1 * NPT + p.l2], v2 = tv[2][i2 * NPT + p.l3];
+    double d0 = td[0][i0 * NPT + p.l1], d1 = td[1][i1 * NPT + p.l2], d2 = td[2][i2 * NPT + p.l3];
+    double gr[3] = {d0 * v1 * v2, v0 * d1 * v2, v0 * v1 * d2};
+    if (SPACE == 0) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) out[c] = gr[0] * gp[c] + gr[1] * gp[3 + c] + gr[2] * gp[6 + c];
+    } else {
+        int cp = desc_comp(d);
+        double sg = desc_neg(d) ? -1.0 : 1.0;
+        double val = sg * v0 * v1 * v2;
+        // curl of val e_cp in reference coordinates: grad(val) x e_cp
+        double cu[3];
+        cu[cp] = 0.0;
+        cu[(cp + 1) % 3] = sg * gr[(cp + 2) % 3];
+        cu[(cp + 2) % 3] = -sg * gr[(cp + 1) % 3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            out[c] = gp[3 * c] * cu[0] + gp[3 * c + 1] * cu[1] + gp[3 * c + 2] * cu[2];
+            out[3 + c] = val * gp[9 + 3 * cp + c];
+        }
+    }
+}
+
+// load-vector contribution of row d at one point
+template <int SPACE>
+__device__ inline double load_entry(unsigned d, Pt p, const double (*tv)[NTAB * NPT], const double *gp)
+{
+    double v = tv[0][desc_ix(d, 0) * NPT + p.l1] * tv[1][desc_ix(d, 1) * NPT + p.l2] *
+               tv[2][desc_ix(d, 2) * NPT + p.l3];
+    if (SPACE == 0) return v * gp[9];
+    return (desc_neg(d) ? -v : v) * gp[18 + desc_comp(d)];
+}
+
+// element header: orders, counts, quadrature sizes (thread 0), plus status on
+// failure.  Returns false (uniformly) if the element is invalid.
+template <int SPACE>
+__device__ bool element_setup(const ElemArgs &a, int e, ElemShared &es)
+{
+    if (threadIdx.x == 0) {
+        ElemShared s;
+        s.ok = decode_orders(a.norder + 19 * (size_t)e, a.nori_e + 12 * (size_t)e,
+                             a.nori_f + 6 * (size_t)e, s.o);
+        s.n = s.nb = s.nint = s.nh = 0;
+        s.nq[0] = s.nq[1] = s.nq[2] = 1;
+        if (s.ok) {
+            dof_counts(SPACE, s.o, s.n, s.nb);
+            int nbh;
+            dof_counts(0, s.o, s.nh, nbh);
+            quad_counts(s.o, s.nq);
+            s.nint = s.nq[0] * s.nq[1] * s.nq[2];
+            if (a.ngeo > s.nh) s.ok = 0;
+        }
+        s.negjac = 0;
+        es = s;
+    }
+    __syncthreads();
+    return es.ok != 0;
+}
+
+// write an accumulator tile (rows R.., cols C..) of a symmetric matrix into
+// column-major `out` (leading dimension ld, valid extent lim x lim), both the
+// tile and its mirror; `stage` is this wave's 16 x 17 LDS scratch.
+template <typename V, typename S, typename M>
+__device__ inline void write_sym_tile(V acc, int R, int C, int lim, S *out, size_t ld, S *stage, int lane)
+{
+    int rr = C + (lane & 15);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        int cc = R + M::row(lane, j);
+        if (rr < lim && cc < lim) out[(size_t)cc * ld + rr] = (S)acc[j];
+    }
+    if (R != C) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) stage[M::row(lane, j) * 17 + (lane & 15)] = (S)acc[j];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        int r2 = R + (lane & 15);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            int c2 = C + 4 * (lane >> 4) + j;
+            S v = stage[(lane & 15) * 17 + 4 * (lane >> 4) + j];
+            if (r2 < lim && c2 < lim) out[(size_t)c2 * ld + r2] = v;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
+__device__ inline void wave_sync_lds()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// General element kernel: one workgroup per (element, 128 x 128 block of K).
 template <typename T, int SPACE>
 __global__ __launch_bounds__(WG) void elem_kernel(ElemArgs a)
 {
@@ -181,34 +348,14 @@ __global__ __launch_bounds__(WG) void elem_kernel(ElemArgs a)
     block_ij(blockIdx.y, I, J);
     const bool diag = (I == J);
 
-    if (tid == 0) {
-        ElemShared s;
-        s.ok = decode_orders(a.norder + 19 * (size_t)e, a.nori_e + 12 * (size_t)e,
-                             a.nori_f + 6 * (size_t)e, s.o);
-        s.n = s.nb = s.nint = s.nh = 0;
-        s.nq[0] = s.nq[1] = s.nq[2] = 1;
-        if (s.ok) {
-            dof_counts(SPACE, s.o, s.n, s.nb);
-            int nbh;
-            dof_counts(0, s.o, s.nh, nbh);
-            quad_counts(s.o, s.nq);
-            s.nint = s.nq[0] * s.nq[1] * s.nq[2];
-            if (a.ngeo > s.nh) s.ok = 0;
-        }
-        s.negjac = 0;
-        es = s;
-    }
-    __syncthreads();
-    if (!es.ok) {
+    if (!element_setup<SPACE>(a, e, es)) {
         if (tid == 0 && blockIdx.y == 0 && a.status) a.status[e] = HP3D_E_ORDER;
         return;
     }
     const int n = es.n, nint = es.nint;
     if (I * BR >= n || J * BR >= n) return; // block beyond this element's size
 
-    // 1D tables: one wave per direction
     if (wave < 3) fill_tables(es.nq[wave], tv[wave], td[wave]);
-    // descriptors of the block's rows and of the geometry dofs
     for (int r = tid; r < 2 * BR; r += WG) {
         int g = (r < BR ? I * BR + r : J * BR + r - BR);
         dsp[r] = g < n ? (SPACE == 0 ? desc_h1(g, es.o) : desc_hcurl(g, es.o)) : 0u;
@@ -235,122 +382,30 @@ __global__ __launch_bounds__(WG) void elem_kernel(ElemArgs a)
     }
     const int nrows = diag ? BR : 2 * BR;
     const bool do_load = diag && a.bout != nullptr;
-    double bsum = 0.0; // load of row I*BR + tid (diagonal blocks, tid < BR)
+    double bsum = 0.0;
 
     for (int g0 = 0; g0 < nint; g0 += GQ) {
-        // geometry of points g0 .. g0+GQ-1 (FP64 always): x, J, det J, J^-1
-        for (int lq = tid; lq < GQ && g0 + lq < nint; lq += WG) {
-            Pt p = point_ix(g0 + lq, es.nq);
-            double x[3] = {0, 0, 0}, jm[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-            for (int k = 0; k < a.ngeo; ++k) {
-                unsigned d = dgeo[k];
-                int i0 = desc_ix(d, 0), i1 = desc_ix(d, 1), i2 = desc_ix(d, 2);
-                double v0 = tv[0][i0 * NPT + p.l1], v1 = tv[1][i1 * NPT + p.l2], v2 = tv[2][i2 * NPT + p.l3];
-                double g0_ = td[0][i0 * NPT + p.l1] * v1 * v2;
-                double g1_ = v0 * td[1][i1 * NPT + p.l2] * v2;
-                double g2_ = v0 * v1 * td[2][i2 * NPT + p.l3];
-                double ph = v0 * v1 * v2;
-                for (int c = 0; c < 3; ++c) {
-                    double xc = xn[3 * k + c];
-                    x[c] += xc * ph;
-                    jm[c][0] += xc * g0_;
-                    jm[c][1] += xc * g1_;
-                    jm[c][2] += xc * g2_;
-                }
-            }
-            // determinant and cofactor inverse (geom.F90:57-113)
-            double det = jm[0][0] * jm[1][1] * jm[2][2] + jm[1][0] * jm[2][1] * jm[0][2] +
-                         jm[2][0] * jm[0][1] * jm[1][2] - jm[2][0] * jm[1][1] * jm[0][2] -
-                         jm[0][0] * jm[2][1] * jm[1][2] - jm[1][0] * jm[0][1] * jm[2][2];
-            if (det < 0.0) es.negjac = 1;
-            double inv[3][3]; // inv[i][j] = d xi_i / d x_j
-            inv[0][0] = (jm[1][1] * jm[2][2] - jm[2][1] * jm[1][2]) / det;
-            inv[1][0] = (-jm[1][0] * jm[2][2] + jm[2][0] * jm[1][2]) / det;
-            inv[2][0] = (jm[1][0] * jm[2][1] - jm[2][0] * jm[1][1]) / det;
-            inv[0][1] = (jm[2][1] * jm[0][2] - jm[0][1] * jm[2][2]) / det;
-            inv[1][1] = (jm[0][0] * jm[2][2] - jm[2][0] * jm[0][2]) / det;
-            inv[2][1] = (-jm[0][0] * jm[2][1] + jm[2][0] * jm[0][1]) / det;
-            inv[0][2] = (jm[0][1] * jm[1][2] - jm[1][1] * jm[0][2]) / det;
-            inv[1][2] = (-jm[0][0] * jm[1][2] + jm[1][0] * jm[0][2]) / det;
-            inv[2][2] = (jm[0][0] * jm[1][1] - jm[1][0] * jm[0][1]) / det;
-            double w = (0.5 * c_gw[es.nq[0] - 1][p.l1]) * (0.5 * c_gw[es.nq[1] - 1][p.l2]) *
-                       (0.5 * c_gw[es.nq[2] - 1][p.l3]);
-            double wd = det * w;
-            double f[3];
-            getf(a.load, x, f);
-            double *gp = geo + lq * GS;
-            if (SPACE == 0) {
-                double s = sqrt(coef * wd);
-                for (int i = 0; i < 3; ++i)
-                    for (int j = 0; j < 3; ++j) gp[3 * i + j] = inv[i][j] * s; // [ref][phys]
-                gp[9] = f[0] * wd;
-            } else {
-                double sc = sqrt(wd) / det, sm = sqrt(coef * wd); // mu = 1
-                for (int i = 0; i < 3; ++i)
-                    for (int j = 0; j < 3; ++j) {
-                        gp[3 * i + j] = jm[i][j] * sc;      // curl: [phys][ref]
-                        gp[9 + 3 * i + j] = inv[i][j] * sm; // mass: [ref][phys]
-                    }
-                for (int c = 0; c < 3; ++c)
-                    gp[18 + c] = (inv[c][0] * f[0] + inv[c][1] * f[1] + inv[c][2] * f[2]) * wd;
-            }
-        }
+        for (int lq = tid; lq < GQ && g0 + lq < nint; lq += WG)
+            if (point_geometry<SPACE>(es, g0 + lq, tv, td, dgeo, xn, a.ngeo, coef, a.load, geo + lq * GS))
+                es.negjac = 1;
         __syncthreads();
-
-        // load vector rows of a diagonal block
-        if (do_load && tid < BR && I * BR + tid < n) {
-            unsigned d = dsp[tid];
-            int i0 = desc_ix(d, 0), i1 = desc_ix(d, 1), i2 = desc_ix(d, 2);
-            for (int lq = 0; lq < GQ && g0 + lq < nint; ++lq) {
-                Pt p = point_ix(g0 + lq, es.nq);
-                double v = tv[0][i0 * NPT + p.l1] * tv[1][i1 * NPT + p.l2] * tv[2][i2 * NPT + p.l3];
-                if (SPACE == 0) bsum += v * geo[lq * GS + 9];
-                else bsum += (desc_neg(d) ? -v : v) * geo[lq * GS + 18 + desc_comp(d)];
-            }
-        }
-
         const int gend = min(GQ, nint - g0);
+        if (do_load && tid < BR && I * BR + tid < n)
+            for (int lq = 0; lq < gend; ++lq)
+                bsum += load_entry<SPACE>(dsp[tid], point_ix(g0 + lq, es.nq), tv, geo + lq * GS);
         for (int l0 = 0; l0 < gend; l0 += QC) {
-            // A chunk: column NC*q + c, row r
             for (int it = tid; it < nrows * QC; it += WG) {
                 int r = it % nrows, q = it / nrows, lq = l0 + q;
                 int g = (r < BR ? I * BR + r : J * BR + r - BR);
                 double out[NC];
 #pragma unroll
                 for (int c = 0; c < NC; ++c) out[c] = 0.0;
-                if (g < n && lq < gend) {
-                    Pt p = point_ix(g0 + lq, es.nq);
-                    unsigned d = dsp[r];
-                    int i0 = desc_ix(d, 0), i1 = desc_ix(d, 1), i2 = desc_ix(d, 2);
-                    double v0 = tv[0][i0 * NPT + p.l1], v1 = tv[1][i1 * NPT + p.l2], v2 = tv[2][i2 * NPT + p.l3];
-                    double d0 = td[0][i0 * NPT + p.l1], d1 = td[1][i1 * NPT + p.l2], d2 = td[2][i2 * NPT + p.l3];
-                    double gr[3] = {d0 * v1 * v2, v0 * d1 * v2, v0 * v1 * d2};
-                    const double *gp = geo + lq * GS;
-                    if (SPACE == 0) {
-#pragma unroll
-                        for (int c = 0; c < 3; ++c)
-                            out[c] = gr[0] * gp[c] + gr[1] * gp[3 + c] + gr[2] * gp[6 + c];
-                    } else {
-                        int cp = desc_comp(d);
-                        double sg = desc_neg(d) ? -1.0 : 1.0;
-                        double val = sg * v0 * v1 * v2;
-                        // curl of val e_cp in reference coordinates: grad(val) x e_cp
-                        double cu[3];
-                        cu[cp] = 0.0;
-                        cu[(cp + 1) % 3] = sg * gr[(cp + 2) % 3];
-                        cu[(cp + 2) % 3] = -sg * gr[(cp + 1) % 3];
-#pragma unroll
-                        for (int c = 0; c < 3; ++c) {
-                            out[c] = gp[3 * c] * cu[0] + gp[3 * c + 1] * cu[1] + gp[3 * c + 2] * cu[2];
-                            out[3 + c] = val * gp[9 + 3 * cp + c];
-                        }
-                    }
-                }
+                if (g < n && lq < gend)
+                    a_entries<SPACE>(dsp[r], point_ix(g0 + lq, es.nq), tv, td, geo + lq * GS, out);
 #pragma unroll
                 for (int c = 0; c < NC; ++c) ac[(NC * q + c) * LDA + r] = (T)out[c];
             }
             __syncthreads();
-            // MFMA over the chunk's KC columns, 4 per step
 #pragma unroll
             for (int k0 = 0; k0 < KC; k0 += 4) {
                 const T *col = ac + (k0 + (lane >> 4)) * LDA;
@@ -369,9 +424,6 @@ __global__ __launch_bounds__(WG) void elem_kernel(ElemArgs a)
     if (do_load && tid < BR && I * BR + tid < n) a.bout[a.off_v[e] + I * BR + tid] = bsum;
     if (tid == 0 && blockIdx.y == 0 && a.status) a.status[e] = es.negjac ? HP3D_NEG_JACOBIAN : 0;
 
-    // write-out.  Lane holds D[row(lane,j)][lane%16] of tile (ti,tj), i.e.
-    // K(R + row, C + lane%16).  By symmetry that is K(C + lane%16, R + row):
-    // a coalesced column segment of the mirrored tile.
     T *kout = (T *)a.kout + a.off_k[e];
     const size_t ld = a.ld_k > 0 ? (size_t)a.ld_k : (size_t)n;
     T *stage = ac + wave * (16 * 17);
@@ -379,32 +431,237 @@ __global__ __launch_bounds__(WG) void elem_kernel(ElemArgs a)
     for (int t = 0; t < 16; ++t) {
         if (!(valid & (1u << t))) continue;
         int ti = t < 8 ? tiA : tiB, tj = t & 7;
-        int R = I * BR + ti * 16, C = J * BR + tj * 16;
-        int rr = C + (lane & 15);
+        write_sym_tile<acc_t, T, M>(acc[t], I * BR + ti * 16, J * BR + tj * 16, n, kout, ld, stage, lane);
+    }
+}
+
+// Single-block element kernel (N <= 128): K accumulates in registers over the
+// whole quadrature, then (when the caller asks for it and nb <= NBF) the
+// bubble dofs are condensed in place of a separate pass:
+//   Abb = L L^T (FP64, LDS), Y = L^-1 [Abi | Bb], Aii -= Y^T Y, Bi -= Y^T y_b
+//   (FP64 MFMA on the register tiles), X = L^-T Y = [ASchur | BSchur]
+// -- the algebra of stc_fwd_herm (stc.F90:338-416) with Aib Abb^-1 Abi
+// written as (L^-1 Abi)^T (L^-1 Abi).
+constexpr int NBF = 32;     // largest bubble block condensed in the fused path
+constexpr int LDS1 = BR + 4; // LDS stride of the single-block kernel's A columns
+struct CondArgs {
+    const long long *off_c, *off_s;
+    double *aii, *bi, *aschur, *bschur;
+};
+
+template <typename T, int SPACE>
+__global__ __launch_bounds__(WG) void elem_single_kernel(ElemArgs a, CondArgs cd)
+{
+    typedef Mfma<T> M;
+    typedef typename M::acc_t acc_t;
+    typedef Mfma<double> MD;
+    constexpr int NC = SPACE == 0 ? 3 : 6;
+    constexpr int QC = KC / NC;
+    constexpr int GQ = 64;
+    constexpr int GS = SPACE == 0 ? 10 : 21;
+    constexpr int NSLOT = 9;
+    constexpr int AC_D = (KC * LDS1 * (int)sizeof(T) + 7) / 8;   // A chunk, in doubles
+    constexpr int KB_D = NBF * LDS1 + 2 * LDS1;                  // bubble rows + two vectors
+    constexpr int SM_D = AC_D > KB_D ? AC_D : KB_D;
+
+    __shared__ ElemShared es;
+    __shared__ double tv[3][NTAB * NPT], td[3][NTAB * NPT];
+    __shared__ unsigned dsp[BR];
+    __shared__ unsigned dgeo[MAXN_H1];
+    __shared__ double geo[GQ * GS];
+    __shared__ double smem[SM_D];
+    __shared__ int s_fail;
+    T *ac = reinterpret_cast<T *>(smem);
+
+    const int e = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (!element_setup<SPACE>(a, e, es)) {
+        if (tid == 0 && a.status) a.status[e] = HP3D_E_ORDER;
+        return;
+    }
+    const int n = es.n, nint = es.nint;
+    if (wave < 3) fill_tables(es.nq[wave], tv[wave], td[wave]);
+    for (int r = tid; r < BR; r += WG) dsp[r] = r < n ? (SPACE == 0 ? desc_h1(r, es.o) : desc_hcurl(r, es.o)) : 0u;
+    for (int k = tid; k < a.ngeo; k += WG) dgeo[k] = desc_h1(k, es.o);
+    if (tid == 0) s_fail = 0;
+    __syncthreads();
+
+    const double *xn = a.xnod + 3 * (size_t)a.ngeo * e;
+    const double coef = a.coef[e];
+    // slots: tile row w (tiles w..7) then tile row 7-w (tiles 7-w..7): 9 per wave
+    const int tiA = wave, tiB = 7 - wave, nA = 8 - wave;
+    acc_t acc[NSLOT];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            int cc = R + M::row(lane, j);
-            if (rr < n && cc < n) kout[(size_t)cc * ld + rr] = acc[t][j];
-        }
-        if (R != C) {
-            // the tile itself, transposed through this wave's LDS stage
+    for (int s = 0; s < NSLOT; ++s) acc[s] = acc_t{0, 0, 0, 0};
+    unsigned valid = 0;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) stage[M::row(lane, j) * 17 + (lane & 15)] = acc[t][j];
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            int r2 = R + (lane & 15);
+    for (int s = 0; s < NSLOT; ++s) {
+        int ti = s < nA ? tiA : tiB, tj = s < nA ? tiA + s : s - 1;
+        valid |= ((ti * 16 < n && tj * 16 < n) ? 1u : 0u) << s;
+    }
+    double bsum = 0.0;
+
+    for (int g0 = 0; g0 < nint; g0 += GQ) {
+        for (int lq = tid; lq < GQ && g0 + lq < nint; lq += WG)
+            if (point_geometry<SPACE>(es, g0 + lq, tv, td, dgeo, xn, a.ngeo, coef, a.load, geo + lq * GS))
+                es.negjac = 1;
+        __syncthreads();
+        const int gend = min(GQ, nint - g0);
+        if (tid < n)
+            for (int lq = 0; lq < gend; ++lq)
+                bsum += load_entry<SPACE>(dsp[tid], point_ix(g0 + lq, es.nq), tv, geo + lq * GS);
+        for (int l0 = 0; l0 < gend; l0 += QC) {
+            for (int it = tid; it < BR * QC; it += WG) {
+                int r = it % BR, q = it / BR, lq = l0 + q;
+                double out[NC];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                int c2 = C + 4 * (lane >> 4) + j;
-                T v = stage[(lane & 15) * 17 + 4 * (lane >> 4) + j];
-                if (r2 < n && c2 < n) kout[(size_t)c2 * ld + r2] = v;
+                for (int c = 0; c < NC; ++c) out[c] = 0.0;
+                if (r < n && lq < gend)
+                    a_entries<SPACE>(dsp[r], point_ix(g0 + lq, es.nq), tv, td, geo + lq * GS, out);
+#pragma unroll
+                for (int c = 0; c < NC; ++c) ac[(NC * q + c) * LDS1 + r] = (T)out[c];
             }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            __syncthreads();
+#pragma unroll
+            for (int k0 = 0; k0 < KC; k0 += 4) {
+                const T *col = ac + (k0 + (lane >> 4)) * LDS1;
+                T aA = col[tiA * 16 + (lane & 15)], aB = col[tiB * 16 + (lane & 15)];
+                T b[8];
+#pragma unroll
+                for (int tj = 0; tj < 8; ++tj) b[tj] = col[tj * 16 + (lane & 15)];
+#pragma unroll
+                for (int s = 0; s < NSLOT; ++s)
+                    if (valid & (1u << s)) acc[s] = M::mma(s < nA ? aA : aB, b[s < nA ? tiA + s : s - 1], acc[s]);
+            }
+            __syncthreads();
         }
     }
+
+    if (a.bout && tid < n) a.bout[a.off_v[e] + tid] = bsum;
+    // element matrix (optional in the fused form)
+    if (a.kout) {
+        T *kout = (T *)a.kout + a.off_k[e];
+        const size_t ld = a.ld_k > 0 ? (size_t)a.ld_k : (size_t)n;
+        T *stage = ac + wave * (16 * 17);
+#pragma unroll
+        for (int s = 0; s < NSLOT; ++s) {
+            if (!(valid & (1u << s))) continue;
+            int ti = s < nA ? tiA : tiB, tj = s < nA ? tiA + s : s - 1;
+            write_sym_tile<acc_t, T, M>(acc[s], ti * 16, tj * 16, n, kout, ld, stage, lane);
+        }
+    }
+    const int nb = es.nb, ni = n - nb;
+    if (!cd.aii || nb > NBF) {
+        if (tid == 0 && a.status) a.status[e] = es.negjac ? HP3D_NEG_JACOBIAN : 0;
+        return;
+    }
+    __syncthreads(); // the A-chunk area becomes the condensation workspace
+    double *kb = smem;                     // kb[b * LDS1 + c] = K(ni + b, c)
+    double *bv = smem + NBF * LDS1;        // load vector
+    double *yb = bv + LDS1;                // L^-1 Bb, then Abb^-1 Bb
+    double *aii = cd.aii + cd.off_c[e];
+    double *bi = cd.bi + a.off_v[e];
+    if (tid < n) bv[tid] = bsum;
+    // bubble rows of K from the register tiles
+#pragma unroll
+    for (int s = 0; s < NSLOT; ++s) {
+        if (!(valid & (1u << s))) continue;
+        int ti = s < nA ? tiA : tiB, tj = s < nA ? tiA + s : s - 1;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            int r = ti * 16 + M::row(lane, j), c = tj * 16 + (lane & 15);
+            double v = (double)acc[s][j];
+            if (r < n && c < n) {
+                if (c >= ni) kb[(c - ni) * LDS1 + r] = v;
+                if (r >= ni) kb[(r - ni) * LDS1 + c] = v;
+            }
+        }
+    }
+    __syncthreads();
+    if (nb > 0 && wave == 0) { // Cholesky of Abb (lower, in kb[i][ni + j]) by one wave
+        for (int j = 0; j < nb; ++j) {
+            double d = kb[j * LDS1 + ni + j];
+            if (!(d > 0.0)) {
+                if (lane == 0) s_fail = 1;
+                d = 1.0;
+            }
+            d = sqrt(d);
+            wave_sync_lds();
+            if (lane == 0) kb[j * LDS1 + ni + j] = d;
+            for (int i = j + 1 + lane; i < nb; i += 64) kb[i * LDS1 + ni + j] /= d;
+            wave_sync_lds();
+            int m = nb - j - 1;
+            for (int idx = lane; idx < m * m; idx += 64) {
+                int i = j + 1 + idx % m, k = j + 1 + idx / m;
+                if (i >= k) kb[i * LDS1 + ni + k] -= kb[i * LDS1 + ni + j] * kb[k * LDS1 + ni + j];
+            }
+            wave_sync_lds();
+        }
+    }
+    __syncthreads();
+    // forward solve Y = L^-1 [Abi | Bb]: one column per thread
+    if (tid <= ni && nb > 0) {
+        const int c = tid;
+        for (int i = 0; i < nb; ++i) {
+            double s = c < ni ? kb[i * LDS1 + c] : bv[ni + i];
+            for (int k = 0; k < i; ++k) s -= kb[i * LDS1 + ni + k] * (c < ni ? kb[k * LDS1 + c] : yb[k]);
+            s /= kb[i * LDS1 + ni + i];
+            if (c < ni) kb[i * LDS1 + c] = s;
+            else yb[i] = s;
+        }
+    }
+    __syncthreads();
+    // Bi = Bb_i - Y^T y_b
+    if (tid < ni) {
+        double s = bv[tid];
+        for (int k = 0; k < nb; ++k) s -= kb[k * LDS1 + tid] * yb[k];
+        bi[tid] = s;
+    }
+    // Aii = Kii - Y^T Y on the register tiles, in FP64
+    __shared__ double stage_d[4 * 16 * 17];
+#pragma unroll
+    for (int s = 0; s < NSLOT; ++s) {
+        if (!(valid & (1u << s))) continue;
+        int ti = s < nA ? tiA : tiB, tj = s < nA ? tiA + s : s - 1;
+        int R = ti * 16, C = tj * 16;
+        if (R >= ni || C >= ni) continue;
+        d4_t da; // the tile in the accumulator's layout, promoted to FP64
+#pragma unroll
+        for (int j = 0; j < 4; ++j) da[j] = (double)acc[s][j];
+        d4_t up = d4_t{0, 0, 0, 0};
+        for (int k0 = 0; k0 < nb; k0 += 4) {
+            int k = k0 + (lane >> 4);
+            int ra = R + (lane & 15), cb = C + (lane & 15);
+            double av = (k < nb && ra < ni) ? kb[k * LDS1 + ra] : 0.0;
+            double bw = (k < nb && cb < ni) ? kb[k * LDS1 + cb] : 0.0;
+            up = MD::mma(av, bw, up);
+        }
+        // up[j] is (Y^T Y)(R + MD::row(lane,j), C + lane%16); bring it to the
+        // accumulator's row order through this wave's stage
+        double *st = stage_d + wave * (16 * 17);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) st[MD::row(lane, j) * 17 + (lane & 15)] = up[j];
+        wave_sync_lds();
+#pragma unroll
+        for (int j = 0; j < 4; ++j) da[j] -= st[M::row(lane, j) * 17 + (lane & 15)];
+        wave_sync_lds();
+        write_sym_tile<d4_t, double, M>(da, R, C, ni, aii, (size_t)ni, st, lane);
+    }
+    __syncthreads();
+    // X = L^-T Y -> ASchur (nb x ni, leading dimension nb), BSchur
+    if (tid <= ni && nb > 0) {
+        const int c = tid;
+        double *xs = c < ni ? cd.aschur + cd.off_s[e] + (size_t)c * nb : cd.bschur + a.off_v[e];
+        for (int i = nb - 1; i >= 0; --i) {
+            double s = c < ni ? kb[i * LDS1 + c] : yb[i];
+            for (int k = i + 1; k < nb; ++k) s -= kb[k * LDS1 + ni + i] * (c < ni ? kb[k * LDS1 + c] : yb[k]);
+            s /= kb[i * LDS1 + ni + i];
+            if (c < ni) kb[i * LDS1 + c] = s;
+            else yb[i] = s;
+            xs[i] = s;
+        }
+    }
+    if (tid == 0 && a.status) a.status[e] = s_fail ? HP3D_E_FACTOR : (es.negjac ? HP3D_NEG_JACOBIAN : 0);
 }
 
 // ------------------------------------------------------------ condensation
@@ -466,6 +723,10 @@ __global__ __launch_bounds__(WG) void stc_kernel(StcArgs a)
     }
     if (ni == 0) {
         if (tid == 0 && a.status) a.status[e] = HP3D_E_NOIFACE;
+        return;
+    }
+    if (nb > STC_LDS_NB && nb > a.nbmax) { // workspace sized for fewer bubbles
+        if (tid == 0 && a.status) a.status[e] = HP3D_E_ARG;
         return;
     }
     double *L = nb <= STC_LDS_NB ? lds_l : a.ws + (size_t)e * a.nbmax * a.nbmax;
@@ -592,9 +853,28 @@ static int nblocks_for(int max_ndof)
     return t * (t + 1) / 2;
 }
 
+static int launch_single(int space, int precision, const ElemArgs &args, const CondArgs &cd,
+                         hipStream_t st)
+{
+    dim3 grid(args.nel), block(WG);
+    if (space == HP3D_H1) {
+        if (precision == HP3D_FP64) hipLaunchKernelGGL((elem_single_kernel<double, 0>), grid, block, 0, st, args, cd);
+        else hipLaunchKernelGGL((elem_single_kernel<float, 0>), grid, block, 0, st, args, cd);
+    } else {
+        if (precision == HP3D_FP64) hipLaunchKernelGGL((elem_single_kernel<double, 1>), grid, block, 0, st, args, cd);
+        else hipLaunchKernelGGL((elem_single_kernel<float, 1>), grid, block, 0, st, args, cd);
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 static int launch_elem(int space, int precision, const ElemArgs &args, int max_ndof,
                        hipStream_t st)
 {
+    if (max_ndof <= BR) {
+        CondArgs cd{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+        return launch_single(space, precision, args, cd, st);
+    }
     dim3 grid(args.nel, nblocks_for(max_ndof)), block(WG);
     if (space == HP3D_H1) {
         if (precision == HP3D_FP64) hipLaunchKernelGGL((elem_kernel<double, 0>), grid, block, 0, st, args);
@@ -725,11 +1005,10 @@ int hp3d_gpu_elem_batch(int space, int precision, int nel, const int *norder, co
     return launch_elem(space, precision, a, max_ndof, (hipStream_t)stream);
 }
 
-size_t hp3d_gpu_stc_workspace(int space, int nel, int max_ndof)
+size_t hp3d_gpu_stc_workspace(int nel, int max_nbubble)
 {
-    int nbm = max_bubbles(space, max_ndof);
-    if (nbm <= STC_LDS_NB) return 0;
-    return (size_t)nel * nbm * nbm * sizeof(double);
+    if (nel <= 0 || max_nbubble <= STC_LDS_NB) return 0;
+    return (size_t)nel * max_nbubble * max_nbubble * sizeof(double);
 }
 
 int hp3d_gpu_stc_batch(int space, int precision, int nel, const int *norder,
@@ -737,23 +1016,23 @@ int hp3d_gpu_stc_batch(int space, int precision, int nel, const int *norder,
                        const long long *off_v, const double *bin, const long long *off_c,
                        int ld_c, double *aii, double *bi, const long long *off_s,
                        double *aschur, double *bschur, void *workspace, int *status,
-                       int max_ndof, void *stream)
+                       int max_nbubble, void *stream)
 {
     if ((space != HP3D_H1 && space != HP3D_HCURL) || (precision != HP3D_FP64 && precision != HP3D_FP32))
         return HP3D_E_ARG;
-    if (nel < 0 || max_ndof < 1 || ld_k < 0 || ld_c < 0) return HP3D_E_ARG;
+    if (nel < 0 || max_nbubble < 0 || ld_k < 0 || ld_c < 0) return HP3D_E_ARG;
     if (nel == 0) return 0;
     if (!norder || !off_k || !kin || !off_v || !bin || !off_c || !aii || !bi || !off_s ||
         !aschur || !bschur)
         return HP3D_E_ARG;
-    if (hp3d_gpu_stc_workspace(space, nel, max_ndof) > 0 && !workspace) return HP3D_E_ARG;
+    if (hp3d_gpu_stc_workspace(nel, max_nbubble) > 0 && !workspace) return HP3D_E_ARG;
     if (precision == HP3D_FP32 && (const void *)aii == kin) return HP3D_E_ARG;
     StcArgs a;
     a.space = space;
     a.nel = nel;
     a.ld_k = ld_k;
     a.ld_c = ld_c;
-    a.nbmax = max_bubbles(space, max_ndof);
+    a.nbmax = max_nbubble;
     a.norder = norder;
     a.nsz = nullptr;
     a.off_k = off_k;
@@ -773,6 +1052,55 @@ int hp3d_gpu_stc_batch(int space, int precision, int nel, const int *norder,
     else hipLaunchKernelGGL(stc_kernel<float>, dim3(nel), dim3(WG), 0, st, a);
     HIPCHK(hipGetLastError());
     return 0;
+}
+
+int hp3d_gpu_celem_batch(int space, int precision, int nel, const int *norder, const int *nori_e,
+                         const int *nori_f, const double *xnod, int ngeo, const double *coef,
+                         int load, const long long *off_k, int ld_k, void *kout,
+                         const long long *off_v, double *bout, const long long *off_c,
+                         double *aii, double *bi, const long long *off_s, double *aschur,
+                         double *bschur, void *workspace, int *status, int max_ndof,
+                         int max_nbubble, void *stream)
+{
+    if ((space != HP3D_H1 && space != HP3D_HCURL) || (precision != HP3D_FP64 && precision != HP3D_FP32))
+        return HP3D_E_ARG;
+    if (nel < 0 || ngeo < 1 || ngeo > MAXN_H1 || max_ndof < 1 || max_nbubble < 0 || ld_k < 0)
+        return HP3D_E_ARG;
+    if (nel == 0) return 0;
+    if (!norder || !nori_e || !nori_f || !xnod || !coef || !off_v || !off_c || !aii || !bi ||
+        !off_s || !aschur || !bschur)
+        return HP3D_E_ARG;
+    if (kout && !off_k) return HP3D_E_ARG;
+    if (ld_k > 0 && ld_k < max_ndof) return HP3D_E_ARG;
+    ElemArgs a;
+    a.space = space;
+    a.nel = nel;
+    a.ngeo = ngeo;
+    a.load = load;
+    a.ld_k = ld_k;
+    a.max_blocks_1d = (max_ndof + BR - 1) / BR;
+    a.norder = norder;
+    a.nori_e = nori_e;
+    a.nori_f = nori_f;
+    a.xnod = xnod;
+    a.coef = coef;
+    a.off_k = off_k;
+    a.off_v = off_v;
+    a.kout = kout;
+    a.bout = bout;
+    a.status = status;
+    hipStream_t st = (hipStream_t)stream;
+    if (max_ndof <= BR && max_nbubble <= NBF) {
+        CondArgs cd{off_c, off_s, aii, bi, aschur, bschur};
+        return launch_single(space, precision, a, cd, st);
+    }
+    // two passes: element matrices, then condensation from HBM
+    if (!kout || !bout) return HP3D_E_ARG;
+    int r = launch_elem(space, precision, a, max_ndof, st);
+    if (r) return r;
+    return hp3d_gpu_stc_batch(space, precision, nel, norder, off_k, ld_k, kout, off_v, bout, off_c,
+                              0, aii, bi, off_s, aschur, bschur, workspace, status, max_nbubble,
+                              stream);
 }
 
 int hp3d_gpu_stc_bwd_batch(int space, int nel, const int *norder, const long long *off_s,

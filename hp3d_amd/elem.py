@@ -170,6 +170,7 @@ class ElementBatch:
         self.n, self.nb = [np.asarray(v, dtype=np.int64) for v in ndof(space, norder)]
         self.ni = self.n - self.nb
         self.max_ndof = int(self.n.max()) if self.nel else 1
+        self.max_nbubble = int(self.nb.max()) if self.nel else 0
         self.norder_h = norder
         dev = torch.device("cuda") if device is None else torch.device(device)
         self.device = dev
@@ -198,7 +199,7 @@ class ElementBatch:
         self.off_c = t(self.off_c_h, torch.int64)
         self.off_s = t(self.off_s_h, torch.int64)
         self.status = torch.zeros(self.nel, dtype=torch.int32, device=dev)
-        ws = _lib.load().hp3d_gpu_stc_workspace(space, self.nel, self.max_ndof)
+        ws = _lib.load().hp3d_gpu_stc_workspace(self.nel, self.max_nbubble)
         self.workspace = torch.empty(max(int(ws) // 8, 1), dtype=torch.float64, device=dev)
 
     @classmethod
@@ -247,14 +248,34 @@ class ElementBatch:
             self.space, precision, self.nel, _tptr(self.norder), _tptr(self.off_k), 0,
             _tptr(out["k"]), _tptr(self.off_v), _tptr(out["b"]), _tptr(self.off_c), 0,
             _tptr(out["aii"]), _tptr(out["bi"]), _tptr(self.off_s), _tptr(out["aschur"]),
-            _tptr(out["bschur"]), _tptr(self.workspace), _tptr(self.status), self.max_ndof,
+            _tptr(out["bschur"]), _tptr(self.workspace), _tptr(self.status), self.max_nbubble,
             ctypes.c_void_p(st))
         check(rc, "hp3d_gpu_stc_batch")
 
-    def run(self, precision=FP64, want_k=True, want_stc=True, out=None):
+    def celem(self, out, precision=FP64, stream=None):
+        """Element matrices and condensation in one launch where the element
+        fits one block (fused), else two (no sync).  out['k'] / out['b'] may be
+        absent in the fused case."""
+        import torch
+
+        lib = _lib.load()
+        st = torch.cuda.current_stream().cuda_stream if stream is None else stream
+        rc = lib.hp3d_gpu_celem_batch(
+            self.space, precision, self.nel, _tptr(self.norder), _tptr(self.nori_e),
+            _tptr(self.nori_f), _tptr(self.xnod), self.ngeo, _tptr(self.coef), self.load,
+            _tptr(self.off_k), 0, _tptr(out.get("k")), _tptr(self.off_v), _tptr(out.get("b")),
+            _tptr(self.off_c), _tptr(out["aii"]), _tptr(out["bi"]), _tptr(self.off_s),
+            _tptr(out["aschur"]), _tptr(out["bschur"]), _tptr(self.workspace),
+            _tptr(self.status), self.max_ndof, self.max_nbubble, ctypes.c_void_p(st))
+        check(rc, "hp3d_gpu_celem_batch")
+
+    def run(self, precision=FP64, want_k=True, want_stc=True, out=None, fused=True):
         """Element matrices and (optionally) condensation; returns the outputs."""
         if out is None:
-            out = self.alloc_outputs(precision, True, want_stc)
+            out = self.alloc_outputs(precision, want_k or not fused, want_stc)
+        if want_stc and fused:
+            self.celem(out, precision)
+            return out
         self.element_matrices(out, precision)
         if want_stc:
             self.condense(out, precision)

@@ -138,15 +138,38 @@ int hp3d_gpu_elem_batch(int space, int precision, int nel, const int *norder, co
  *        precision is FP64, off_c == off_k and ld_c == ld_k: in place as ALOC)
  *   bi   at off_v[e]: condensed load (may alias bin)
  *   aschur at off_s[e] (nb_e x ni_e, leading dimension nb_e), bschur at off_v[e]
- *   workspace: device memory of hp3d_gpu_stc_workspace(...) bytes
+ *   workspace: device memory of hp3d_gpu_stc_workspace(nel, max_nbubble) bytes
+ *   max_nbubble bounds nb_e over the batch (an element above it reports
+ *   HP3D_E_ARG in status[e])
  */
 int hp3d_gpu_stc_batch(int space, int precision, int nel, const int *norder,
                        const long long *off_k, int ld_k, const void *kin,
                        const long long *off_v, const double *bin, const long long *off_c,
                        int ld_c, double *aii, double *bi, const long long *off_s,
                        double *aschur, double *bschur, void *workspace, int *status,
-                       int max_ndof, void *stream);
-size_t hp3d_gpu_stc_workspace(int space, int nel, int max_ndof);
+                       int max_nbubble, void *stream);
+/* bytes of device workspace the condensation of nel elements with at most
+ * max_nbubble bubble dofs needs (0 when their bubble blocks fit in LDS) */
+size_t hp3d_gpu_stc_workspace(int nel, int max_nbubble);
+
+/*
+ * Element matrices, loads and static condensation in one call (the work of
+ * celem_systemI.F90:523-534 for nel elements; DEVICE pointers).  Arguments as
+ * in hp3d_gpu_elem_batch and hp3d_gpu_stc_batch; max_nbubble bounds nb_e.
+ * When max_ndof <= 128 and max_nbubble <= 32 the condensation runs inside the
+ * element kernel on the register-resident K (no second pass over HBM); kout
+ * and bout may then be NULL (K and the full load are not written).  Otherwise
+ * K and the load are written and condensed by a second kernel; kout, bout and
+ * the hp3d_gpu_stc_workspace(...) workspace are then required.  The condensed
+ * matrix is packed (leading dimension ni_e).
+ */
+int hp3d_gpu_celem_batch(int space, int precision, int nel, const int *norder, const int *nori_e,
+                         const int *nori_f, const double *xnod, int ngeo, const double *coef,
+                         int load, const long long *off_k, int ld_k, void *kout,
+                         const long long *off_v, double *bout, const long long *off_c,
+                         double *aii, double *bi, const long long *off_s, double *aschur,
+                         double *bschur, void *workspace, int *status, int max_ndof,
+                         int max_nbubble, void *stream);
 
 /*
  * Back substitution for nel elements (stc_bwd, stc.F90:664-679):

@@ -97,6 +97,34 @@ def test_hcurl_fp64(gpu, p):
     compare(batch, out, ref, TOL64, 1e-11)
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_fused_matches_two_pass(gpu, precision):
+    """The single-launch form (condensation on the register-resident K) and the
+    two-pass form (element kernel, then stc_kernel from HBM) agree."""
+    import torch
+
+    import hp3d_amd
+
+    m = mesh.structured((3, 2, 2), porder=4, perturb=0.15, mesh_seed=2, perm_seed=5,
+                        coef=np.exp(np.random.default_rng(0).normal(size=12)))
+    batch = hp3d_amd.ElementBatch.from_mesh(0, m)
+    o1 = batch.run(precision, fused=True)
+    o2 = batch.run(precision, fused=False)
+    o3 = batch.run(precision, want_k=False, fused=True)  # K not written at all
+    torch.cuda.synchronize()
+    batch.check_status()
+    assert "k" not in o3
+    for e in range(batch.nel):
+        for o in (o1, o3):
+            assert rel(batch.aii_of(o, e), batch.aii_of(o2, e)) < 1e-13
+            assert rel(batch.aschur_of(o, e), batch.aschur_of(o2, e)) < 1e-12
+            nb = int(batch.nb[e])
+            assert rel(batch.vec_of(o, "bschur", e, nb), batch.vec_of(o2, "bschur", e, nb)) < 1e-12
+            ni = int(batch.ni[e])
+            assert rel(batch.vec_of(o, "bi", e, ni), batch.vec_of(o2, "bi", e, ni)) < 1e-13
+        assert rel(batch.k_of(o1, e), batch.k_of(o2, e)) == 0.0
+
+
 def test_dropin_elem_and_stc(gpu):
     import hp3d_amd
 

@@ -1,0 +1,373 @@
+"""Pins the oracle (oracle/hp3d_oracle.c) before it is trusted as the checker.
+
+1. Against the reference's own routines: the committed fixtures of
+   tests/golden/shape_ref.npz (written by tests/golden/make_golden.py from the
+   reference Fortran compiled into oracle/_ref), and oracle/_ref directly where
+   it is built -- bitwise for Legendre recurrences, Gauss tables, geom and the
+   H1 / H(curl) hexahedron shape functions.
+2. Against what mathematics fixes: numpy Legendre closed forms and Gauss rules,
+   partition of unity, polynomial reproduction, the exact sequence, derivative
+   consistency by finite differences, vanishing traces, the closed-form p=1
+   cube Laplacian, energy identities by an independent quadrature, scipy
+   LAPACK for condensation and the Schur identity.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "shape_ref.npz")
+
+
+def uniform(p):
+    return [p] * 12 + [p * 11] * 6 + [p * 111]
+
+
+def aniso(px, py, pz):
+    return [px, py, px, py, px, py, px, py, pz, pz, pz, pz,
+            px * 10 + py, px * 10 + py, px * 10 + pz, py * 10 + pz, px * 10 + pz, py * 10 + pz,
+            px * 100 + py * 10 + pz]
+
+
+Z12, Z6 = [0] * 12, [0] * 6
+
+
+# ----------------------------------------------------------- 1. reference pins
+
+def test_fixtures_shape_functions_bitwise():
+    g = np.load(GOLD)
+    for t in range(g["xi"].shape[0]):
+        xi, no, ne, nf = g["xi"][t], g["norder"][t], g["nori_e"][t], g["nori_f"][t]
+        s, gr = oracle.shape3dh(xi, no, ne, nf)
+        n = s.size
+        assert np.array_equal(s, g["h1_shap"][t][:n]) and np.array_equal(gr, g["h1_grad"][t][:n])
+        assert not g["h1_shap"][t][n:].any()
+        s, c = oracle.shape3de(xi, no, ne, nf)
+        n = s.shape[0]
+        assert np.array_equal(s, g["hc_shap"][t][:n]) and np.array_equal(c, g["hc_curl"][t][:n])
+        assert not g["hc_shap"][t][n:].any()
+
+
+def test_fixtures_legendre_gauss_geom_bitwise():
+    g = np.load(GOLD)
+    for i, x in enumerate(g["leg_x"]):
+        assert np.array_equal(oracle.poly_legendre(x, 0.75 + x, 6), g["leg_p"][i])
+        assert np.array_equal(oracle.poly_ilegendre(x, 6)[0][2:], g["ileg_l"][i][2:])
+    for n in range(1, 11):
+        x, w = oracle.gauss1d(n)
+        assert np.array_equal(x, g["gauss_x"][n - 1, :n]) and np.array_equal(w, g["gauss_w"][n - 1, :n])
+    for a, inv, det, fl in zip(g["geom_in"], g["geom_inv"], g["geom_det"], g["geom_flag"]):
+        b, r, f = oracle.geom(a)
+        assert np.array_equal(b, inv) and r == det and f == fl
+    assert g["geom_flag"].sum() == 1
+
+
+@pytest.mark.skipif(oracle.ref() is None, reason="oracle/_ref not built (needs the reference tree)")
+def test_live_reference_shape_functions_bitwise():
+    rng = np.random.default_rng(1)
+    for t in range(300):
+        no = np.zeros(19, np.int32)
+        no[:12] = rng.integers(1, 7, 12)
+        no[12:18] = rng.integers(1, 7, 6) * 10 + rng.integers(1, 7, 6)
+        p = rng.integers(1, 7, 3)
+        no[18] = p[0] * 100 + p[1] * 10 + p[2]
+        ne, nf = rng.integers(0, 2, 12), rng.integers(0, 8, 6)
+        xi = rng.random(3)
+        a, b = oracle.shape3dh(xi, no, ne, nf), oracle.ref_shape3dh(xi, no, ne, nf)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+        a, b = oracle.shape3de(xi, no, ne, nf), oracle.ref_shape3de(xi, no, ne, nf)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+@pytest.mark.skipif(oracle.ref() is None, reason="oracle/_ref not built (needs the reference tree)")
+def test_live_reference_tables_bitwise():
+    for n in range(1, 11):
+        assert all(np.array_equal(u, v) for u, v in zip(oracle.gauss1d(n), oracle.ref_gauss1d(n)))
+    for x in np.linspace(0, 1, 13):
+        for t in (1.0, 0.5, 1.3):
+            assert np.array_equal(oracle.poly_legendre(x, t, 6), oracle.ref_poly_legendre(x, t, 6))
+        assert np.array_equal(oracle.poly_ilegendre(x, 6)[0][2:], oracle.ref_poly_ilegendre(x, 6)[0][2:])
+
+
+# --------------------------------------------------------- 2. mathematics pins
+
+def test_legendre_closed_forms():
+    from numpy.polynomial import legendre as L
+
+    for x in np.linspace(0, 1, 11):
+        for t in (1.0, 0.7):
+            p = oracle.poly_legendre(x, t, 6)
+            for i in range(7):
+                ref = L.legval((2 * x - t) / t, [0] * i + [1]) * t ** i
+                assert abs(p[i] - ref) < 1e-14 * max(1.0, abs(ref))
+        l_, dp = oracle.poly_ilegendre(x, 6)
+        for i in range(2, 7):
+            pi = L.legval(2 * x - 1, [0] * i + [1])
+            pim2 = L.legval(2 * x - 1, [0] * (i - 2) + [1])
+            assert abs(l_[i] - (pi - pim2) / (2 * (2 * i - 1))) < 1e-15
+            # derivative identity d/dx L_i(2x-1) = P_{i-1}(2x-1)
+            assert abs(dp[i - 1] - L.legval(2 * x - 1, [0] * (i - 1) + [1])) < 1e-14
+
+
+def test_gauss_rules():
+    for n in range(1, 11):
+        x, w = oracle.gauss1d(n)
+        xr, wr = np.polynomial.legendre.leggauss(n)
+        tol = 1e-14 if n < 10 else 2e-16 * 10
+        assert np.allclose(x, 0.5 * (xr + 1), atol=tol, rtol=0)
+        assert np.allclose(w, 0.5 * wr, atol=tol, rtol=0)
+        assert abs(w.sum() - 1.0) < 5e-15
+        for k in range(2 * n):  # exact to degree 2n-1
+            assert abs(np.dot(w, x ** k) - 1.0 / (k + 1)) < 5e-14
+
+
+def test_set_3d_int_counts_and_order():
+    no = aniso(2, 3, 4)
+    xi, w = oracle.set_3d_int(no, Z6)
+    assert xi.shape[0] == 3 * 4 * 5 and abs(w.sum() - 1) < 1e-14
+    # l1 fastest (set_3D_int.F90:248-257)
+    assert xi[0, 0] < xi[1, 0] and xi[0, 1] == xi[1, 1]
+    # a swapped face (code 1) exchanges the face's local orders
+    no2 = uniform(2)
+    no2[12] = 25  # face 1: h=2 (x), v=5 (y) in face-node axes
+    assert oracle.set_3d_int(no2, Z6)[0].shape[0] == 3 * 6 * 3
+    assert oracle.set_3d_int(no2, [1, 0, 0, 0, 0, 0])[0].shape[0] == 6 * 3 * 3
+
+
+def grid_points(n=4):
+    g = (np.arange(n) + 0.37) / n
+    return np.array([(a, b, c) for a in g for b in g for c in g])
+
+
+@pytest.mark.parametrize("p", [(1, 1, 1), (2, 3, 4), (4, 4, 4), (6, 5, 3)])
+def test_h1_partition_of_unity_and_reproduction(p):
+    no = aniso(*p)
+    pts = grid_points(6)
+    vals = np.array([oracle.shape3dh(x, no, Z12, Z6)[0] for x in pts])
+    assert np.allclose(vals[:, :8].sum(1), 1.0, atol=1e-15)
+    # Q^{px,py,pz} is reproduced exactly
+    mons = [(a, b, c) for a in range(p[0] + 1) for b in range(p[1] + 1) for c in range(p[2] + 1)]
+    assert vals.shape[1] == len(mons)
+    for a, b, c in mons:
+        f = pts[:, 0] ** a * pts[:, 1] ** b * pts[:, 2] ** c
+        coef, res, *_ = np.linalg.lstsq(vals, f, rcond=None)
+        assert np.max(np.abs(vals @ coef - f)) < 1e-12
+
+
+def test_h1_gradients_and_hcurl_curls_by_finite_differences():
+    rng = np.random.default_rng(3)
+    no = aniso(3, 4, 2)
+    ne, nf = rng.integers(0, 2, 12), rng.integers(0, 8, 6)
+    h = 1e-6
+    for x in rng.uniform(0.1, 0.9, (5, 3)):
+        s, g = oracle.shape3dh(x, no, ne, nf)
+        e, c = oracle.shape3de(x, no, ne, nf)
+        dE = np.zeros((3,) + e.shape)
+        for d in range(3):
+            xp, xm = x.copy(), x.copy()
+            xp[d] += h
+            xm[d] -= h
+            fd = (oracle.shape3dh(xp, no, ne, nf)[0] - oracle.shape3dh(xm, no, ne, nf)[0]) / (2 * h)
+            assert np.max(np.abs(fd - g[:, d])) < 1e-8
+            dE[d] = (oracle.shape3de(xp, no, ne, nf)[0] - oracle.shape3de(xm, no, ne, nf)[0]) / (2 * h)
+        curl = np.stack([dE[1][:, 2] - dE[2][:, 1], dE[2][:, 0] - dE[0][:, 2],
+                         dE[0][:, 1] - dE[1][:, 0]], 1)
+        assert np.max(np.abs(curl - c)) < 1e-8
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_exact_sequence_grad_h1_in_hcurl(p):
+    rng = np.random.default_rng(p)
+    no = uniform(p)
+    ne, nf = rng.integers(0, 2, 12), rng.integers(0, 8, 6)
+    pts = rng.random((60, 3))
+    G = np.concatenate([oracle.shape3dh(x, no, ne, nf)[1] for x in pts], 0)   # rows: (pt, dof)
+    E = [oracle.shape3de(x, no, ne, nf)[0] for x in pts]                        # (dof, 3)
+    nh, ne_ = G.shape[0] // len(pts), E[0].shape[0]
+    grads = np.stack([oracle.shape3dh(x, no, ne, nf)[1] for x in pts])         # (pt, nh, 3)
+    basis = np.stack(E)                                                        # (pt, nE, 3)
+    A = basis.transpose(0, 2, 1).reshape(-1, ne_)                               # (pt*3, nE)
+    for k in range(nh):
+        f = grads[:, k, :].reshape(-1)
+        coef = np.linalg.lstsq(A, f, rcond=None)[0]
+        assert np.max(np.abs(A @ coef - f)) < 1e-11
+    del G
+
+
+def test_traces_vanish():
+    rng = np.random.default_rng(2)
+    no = uniform(3)
+    ne, nf = rng.integers(0, 2, 12), rng.integers(0, 8, 6)
+    nb = oracle.nbubble_h1(333)
+    for _ in range(10):
+        x = rng.random(3)
+        x[rng.integers(3)] = rng.integers(2)   # a point on the boundary
+        s, _g = oracle.shape3dh(x, no, ne, nf)
+        assert np.max(np.abs(s[-nb:])) < 1e-15   # H1 bubbles
+        e, _c = oracle.shape3de(x, no, ne, nf)
+        nbe = oracle.nbubble_hcurl(333)
+        d = [k for k in range(3) if x[k] in (0.0, 1.0)][0]
+        tang = [k for k in range(3) if k != d]
+        assert np.max(np.abs(e[-nbe:][:, tang])) < 1e-15   # tangential traces of H(curl) bubbles
+
+
+def test_p1_cube_laplacian_closed_form():
+    xn = np.array([(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0),
+                   (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)], float).T
+    k, b, st = oracle.elem_h1(uniform(1), Z12, Z6, xn, load=oracle.LOAD_ONE)
+    ref = np.zeros((8, 8))
+    v = np.array([(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)])
+    for i in range(8):
+        for j in range(8):
+            dist = np.abs(v[i] - v[j]).sum()
+            ref[i, j] = {0: 4, 1: 0, 2: -1, 3: -1}[dist] / 12.0
+    assert np.allclose(k, ref, atol=1e-15) and st == 0
+    assert np.allclose(b, 1 / 8, atol=1e-15)
+    # h-scaling of the 3D Laplacian: K(h x) = h K(x), b(h x) = h^3 b(x)
+    k2, b2, _ = oracle.elem_h1(uniform(1), Z12, Z6, 0.5 * xn, load=oracle.LOAD_ONE)
+    assert np.allclose(k2, 0.5 * k, atol=1e-15) and np.allclose(b2, b / 8, atol=1e-16)
+
+
+def trilinear(xn, xi):
+    s = np.array([(1 - xi[0]) * (1 - xi[1]) * (1 - xi[2]), xi[0] * (1 - xi[1]) * (1 - xi[2]),
+                  xi[0] * xi[1] * (1 - xi[2]), (1 - xi[0]) * xi[1] * (1 - xi[2]),
+                  (1 - xi[0]) * (1 - xi[1]) * xi[2], xi[0] * (1 - xi[1]) * xi[2],
+                  xi[0] * xi[1] * xi[2], (1 - xi[0]) * xi[1] * xi[2]])
+    return xn @ s
+
+
+def jac_fd(xn, xi):
+    """Jacobian of the trilinear map, differentiated by hand."""
+    v = np.array([(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)])
+    f = lambda a, t: t if a else 1 - t
+    df = lambda a: 1.0 if a else -1.0
+    g = np.zeros((8, 3))
+    for k, (a, b, c) in enumerate(v):
+        g[k] = (df(a) * f(b, xi[1]) * f(c, xi[2]), f(a, xi[0]) * df(b) * f(c, xi[2]),
+                f(a, xi[0]) * f(b, xi[1]) * df(c))
+    return xn @ g
+
+
+@pytest.mark.parametrize("p", [2, 4])
+def test_h1_energy_identity_independent_quadrature(p):
+    """u^T K u = kappa * int |grad u|^2 with u = sum c_k phi_k, by numpy's own
+    Gauss rule of the same size and a finite-difference Jacobian."""
+    rng = np.random.default_rng(p)
+    xn = np.array([(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0),
+                   (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)], float).T * 0.3
+    xn = xn + rng.uniform(-0.03, 0.03, xn.shape)
+    ne, nf = rng.integers(0, 2, 12), rng.integers(0, 8, 6)
+    no = uniform(p)
+    kappa = 1.7
+    k, b, st = oracle.elem_h1(no, ne, nf, xn, kappa=kappa, load=oracle.LOAD_SINSINSIN)
+    c = rng.normal(size=k.shape[0])
+    xg, wg = np.polynomial.legendre.leggauss(p + 1)
+    xg, wg = 0.5 * (xg + 1), 0.5 * wg
+    e = 0.0
+    bsum = 0.0
+    for i, a in enumerate(xg):
+        for j, bb in enumerate(xg):
+            for l, cc in enumerate(xg):
+                xi = np.array([a, bb, cc])
+                _s, g = oracle.shape3dh(xi, no, ne, nf)
+                jm = jac_fd(xn, xi)
+                det = np.linalg.det(jm)
+                grad = (g @ np.linalg.inv(jm)).T @ c
+                w = wg[i] * wg[j] * wg[l] * det
+                e += kappa * w * grad @ grad
+                x = trilinear(xn, xi)
+                bsum += w * 3 * np.pi ** 2 * np.prod(np.sin(np.pi * x))
+    assert abs(c @ k @ c - e) < 1e-12 * abs(e)
+    assert abs(b[:8].sum() - bsum) < 1e-12 * abs(bsum)
+    # symmetric, positive semidefinite, constants in the kernel
+    assert np.array_equal(k, k.T)
+    assert np.linalg.eigvalsh(k)[0] > -1e-13 * np.abs(k).max()
+    one = np.zeros(k.shape[0])
+    one[:8] = 1
+    assert np.max(np.abs(k @ one)) < 1e-13 * np.abs(k).max()
+
+
+def test_hcurl_energy_identity_and_gradient_kernel():
+    rng = np.random.default_rng(11)
+    p = 2
+    xn = (np.array([(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0),
+                    (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)], float).T * 0.5
+          + rng.uniform(-0.04, 0.04, (3, 8)))
+    ne, nf = rng.integers(0, 2, 12), rng.integers(0, 8, 6)
+    no = uniform(p)
+    eps = 2.5
+    k, b, st = oracle.elem_hcurl(no, ne, nf, xn, mu=1.0, eps=eps)
+    c = rng.normal(size=k.shape[0])
+    xg, wg = np.polynomial.legendre.leggauss(p + 1)
+    xg, wg = 0.5 * (xg + 1), 0.5 * wg
+    e = 0.0
+    for i, a in enumerate(xg):
+        for j, bb in enumerate(xg):
+            for l, cc in enumerate(xg):
+                xi = np.array([a, bb, cc])
+                s, cu = oracle.shape3de(xi, no, ne, nf)
+                jm = jac_fd(xn, xi)
+                det = np.linalg.det(jm)
+                E = (s @ np.linalg.inv(jm)).T @ c            # covariant Piola
+                C = (jm @ cu.T) @ c / det                   # contravariant Piola
+                e += wg[i] * wg[j] * wg[l] * det * (C @ C + eps * E @ E)
+    assert abs(c @ k @ c - e) < 1e-12 * abs(e)
+    # curl part annihilates gradients: K_curl = K(eps=0) has grad(H1) in its kernel
+    k0, _b, _ = oracle.elem_hcurl(no, ne, nf, xn, mu=1.0, eps=0.0)
+    pts = rng.random((40, 3))
+    A = np.concatenate([oracle.shape3de(x, no, ne, nf)[0] for x in pts], 0)
+    for kk in range(8, 20):  # some H1 edge / face functions
+        f = np.concatenate([oracle.shape3dh(x, no, ne, nf)[1][kk] for x in pts])
+        coef = np.linalg.lstsq(A.reshape(len(pts), -1, 3).transpose(0, 2, 1).reshape(-1, A.shape[0] // len(pts)),
+                               f, rcond=None)[0]
+        assert np.max(np.abs(k0 @ coef)) < 1e-10 * np.abs(k0).max()
+
+
+@pytest.mark.parametrize("herm", [False, True])
+def test_condensation_against_scipy_and_schur_identity(herm):
+    import scipy.linalg as sl
+
+    rng = np.random.default_rng(4)
+    xn = (np.array([(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0),
+                    (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)], float).T
+          + rng.uniform(-0.1, 0.1, (3, 8)))
+    no = uniform(4)
+    k, b, _ = oracle.elem_h1(no, Z12, Z6, xn, kappa=2.0)
+    nb = oracle.nbubble_h1(444)
+    ni = k.shape[0] - nb
+    aii, bi, asc, bsc = oracle.stc_fwd(k, b, nb, herm=herm)
+    lu = sl.lu_factor(k[ni:, ni:])
+    x_ref = sl.lu_solve(lu, k[ni:, :ni])
+    xb_ref = sl.lu_solve(lu, b[ni:])
+    assert np.allclose(asc, x_ref, rtol=0, atol=1e-12 * np.abs(x_ref).max())
+    assert np.allclose(bsc, xb_ref, rtol=0, atol=1e-12 * np.abs(xb_ref).max())
+    s_ref = k[:ni, :ni] - k[:ni, ni:] @ x_ref
+    assert np.allclose(aii, s_ref, rtol=0, atol=1e-12 * np.abs(s_ref).max())
+    # Schur identity: solve the full system with the vertex dofs fixed
+    fix = np.arange(8)
+    free = np.setdiff1d(np.arange(k.shape[0]), fix)
+    u = np.zeros(k.shape[0])
+    u[fix] = rng.normal(size=8)
+    rhs = b[free] - k[np.ix_(free, fix)] @ u[fix]
+    u[free] = np.linalg.solve(k[np.ix_(free, free)], rhs)
+    fi = np.setdiff1d(np.arange(ni), fix)
+    xi = np.zeros(ni)
+    xi[fix] = u[fix]
+    xi[fi] = np.linalg.solve(aii[np.ix_(fi, fi)], bi[fi] - aii[np.ix_(fi, fix)] @ u[fix])
+    assert np.allclose(xi, u[:ni], atol=1e-10 * np.abs(u).max())
+    xb = oracle.stc_bwd(asc, bsc, xi)
+    assert np.allclose(xb, u[ni:], atol=1e-10 * np.abs(u).max())
+
+
+def test_order_check_and_negative_jacobian():
+    no = uniform(2)
+    no[0] = 7
+    with pytest.raises(ValueError):
+        oracle.shape3dh([0.5] * 3, no, Z12, Z6)
+    xn = np.array([(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0),
+                   (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)], float).T
+    xn[2] *= -1  # inverted element
+    _k, _b, st = oracle.elem_h1(uniform(1), Z12, Z6, xn)
+    assert st == 1
