@@ -34,7 +34,8 @@ namespace hp3d {
 constexpr int WG = 256;        // threads per workgroup (4 waves)
 constexpr int BR = 128;        // rows of K per block (8 MFMA tiles)
 constexpr int KC = 24;         // A columns per chunk (8 points H1, 4 points H(curl))
-constexpr int LDA = 2 * BR + 4; // LDS stride of one A column
+constexpr int LDA = 2 * BR + 16; // LDS stride of one A column (+16: the four lane
+                                 // groups of an MFMA operand read hit disjoint banks)
 constexpr int NPT = 8;         // padded max Gauss points per direction
 constexpr int MAXQ = (MAXP + 1) * (MAXP + 1) * (MAXP + 1);
 constexpr int MAXN_H1 = (MAXP + 1) * (MAXP + 1) * (MAXP + 1);
@@ -47,15 +48,16 @@ __constant__ double c_gw[10][10] = HP3D_GAUSS_W;
 // ------------------------------------------------------------------ helpers
 
 // 1D factor tables for one direction at the n Gauss points (values, d/dxi)
-__device__ void fill_tables(int n, double *tv, double *td)
+__device__ __forceinline__ void fill_tables(int n, double *tv, double *td)
 {
-    // tv/td laid out [NTAB][NPT]
+    // tv/td laid out point-major, [NPT][NTAB]: a wave's lanes (rows with
+    // different table entries at the same point) read consecutive words
     for (int q = threadIdx.x % 32; q < n; q += 32) {
         double x = 0.5 * (1.0 + c_gx[n - 1][q]);
-        tv[0 * NPT + q] = 1.0 - x;
-        td[0 * NPT + q] = -1.0;
-        tv[1 * NPT + q] = x;
-        td[1 * NPT + q] = 1.0;
+        tv[q * NTAB + 0] = 1.0 - x;
+        td[q * NTAB + 0] = -1.0;
+        tv[q * NTAB + 1] = x;
+        td[q * NTAB + 1] = 1.0;
         for (int flip = 0; flip < 2; ++flip) {
             // integrated Legendre phi_i(s) = (P_i(2s-1) - P_{i-2}(2s-1)) / (4i-2),
             // d phi_i / ds = P_{i-1}(2s-1)   (PolyILegendre, Idec = .true.)
@@ -66,8 +68,8 @@ __device__ void fill_tables(int n, double *tv, double *td)
             for (int i = 2; i <= MAXP; ++i)
                 p[i] = ((double)(2 * i - 1) * y * p[i - 1] - (double)(i - 1) * p[i - 2]) / (double)i;
             for (int i = 2; i <= MAXP; ++i) {
-                tv[t_phi(i, flip) * NPT + q] = (p[i] - p[i - 2]) / (double)(4 * i - 2);
-                td[t_phi(i, flip) * NPT + q] = flip ? -p[i - 1] : p[i - 1];
+                tv[q * NTAB + t_phi(i, flip)] = (p[i] - p[i - 2]) / (double)(4 * i - 2);
+                td[q * NTAB + t_phi(i, flip)] = flip ? -p[i - 1] : p[i - 1];
             }
             // scaled Legendre P_i(s; t) with t = s0 + s1 evaluated as the
             // reference does (HomLegendre of the oriented pair)
@@ -78,8 +80,8 @@ __device__ void fill_tables(int n, double *tv, double *td)
             for (int i = 2; i < MAXP; ++i)
                 r[i] = ((double)(2 * i - 1) * yy * r[i - 1] - (double)(i - 1) * tt * r[i - 2]) / (double)i;
             for (int i = 0; i < MAXP; ++i) {
-                tv[t_leg(i, flip) * NPT + q] = r[i];
-                td[t_leg(i, flip) * NPT + q] = 0.0;
+                tv[q * NTAB + t_leg(i, flip)] = r[i];
+                td[q * NTAB + t_leg(i, flip)] = 0.0;
             }
         }
     }
@@ -162,7 +164,7 @@ __device__ inline void block_ij(int b, int &I, int &J)
 // weights into GS doubles at gp (geom3D.F90:77-91, geom.F90:57-113,
 // elem_opt.F90:101-122).  Returns true when det J < 0.
 template <int SPACE>
-__device__ bool point_geometry(const ElemShared &es, int l, const double (*tv)[NTAB * NPT],
+__device__ __forceinline__ bool point_geometry(const ElemShared &es, int l, const double (*tv)[NTAB * NPT],
                                const double (*td)[NTAB * NPT], const unsigned *dgeo,
                                const double *xn, int ngeo, double coef, int load, double *gp)
 {
@@ -171,10 +173,10 @@ __device__ bool point_geometry(const ElemShared &es, int l, const double (*tv)[N
     for (int k = 0; k < ngeo; ++k) {
         unsigned d = dgeo[k];
         int i0 = desc_ix(d, 0), i1 = desc_ix(d, 1), i2 = desc_ix(d, 2);
-        double v0 = tv[0][i0 * NPT + p.l1], v1 = tv[1][i1 * NPT + p.l2], v2 = tv[2][i2 * NPT + p.l3];
-        double g0 = td[0][i0 * NPT + p.l1] * v1 * v2;
-        double g1 = v0 * td[1][i1 * NPT + p.l2] * v2;
-        double g2 = v0 * v1 * td[2][i2 * NPT + p.l3];
+        double v0 = tv[0][p.l1 * NTAB + i0], v1 = tv[1][p.l2 * NTAB + i1], v2 = tv[2][p.l3 * NTAB + i2];
+        double g0 = td[0][p.l1 * NTAB + i0] * v1 * v2;
+        double g1 = v0 * td[1][p.l2 * NTAB + i1] * v2;
+        double g2 = v0 * v1 * td[2][p.l3 * NTAB + i2];
         double ph = v0 * v1 * v2;
         for (int c = 0; c < 3; ++c) {
             double xc = xn[3 * k + c];
@@ -223,12 +225,12 @@ __device__ bool point_geometry(const ElemShared &es, int l, const double (*tv)[N
 // the NC entries of row d (descriptor) at point p of the A matrix
 // (A_TEST of elem_opt.F90:122, or the [curl | mass] pair of the H(curl) form)
 template <int SPACE>
-__device__ inline void a_entries(unsigned d, Pt p, const double (*tv)[NTAB * NPT],
+__device__ __forceinline__ void a_entries(unsigned d, Pt p, const double (*tv)[NTAB * NPT],
                                  const double (*td)[NTAB * NPT], const double *gp, double *out)
 {
     int i0 = desc_ix(d, 0), i1 = desc_ix(d, 1), i2 = desc_ix(d, 2);
-    double v0 = tv[0][i0 * NPT + p.l1], v1 = tv[1][i1 * NPT + p.l2], v2 = tv[2][i2 * NPT + p.l3];
-    double d0 = td[0][i0 * NPT + p.l1], d1 = td[1][i1 * NPT + p.l2], d2 = td[2][i2 * NPT + p.l3];
+    double v0 = tv[0][p.l1 * NTAB + i0], v1 = tv[1][p.l2 * NTAB + i1], v2 = tv[2][p.l3 * NTAB + i2];
+    double d0 = td[0][p.l1 * NTAB + i0], d1 = td[1][p.l2 * NTAB + i1], d2 = td[2][p.l3 * NTAB + i2];
     double gr[3] = {d0 * v1 * v2, v0 * d1 * v2, v0 * v1 * d2};
     if (SPACE == 0) {
 #pragma unroll
@@ -250,12 +252,46 @@ __device__ inline void a_entries(unsigned d, Pt p, const double (*tv)[NTAB * NPT
     }
 }
 
+// A entries and the load contribution of row d at one point (one pass over
+// the 1D tables for both)
+template <int SPACE>
+__device__ __forceinline__ double a_entries_load(unsigned d, Pt p, const double (*tv)[NTAB * NPT],
+                                                 const double (*td)[NTAB * NPT], const double *gp,
+                                                 double *out)
+{
+    int i0 = desc_ix(d, 0), i1 = desc_ix(d, 1), i2 = desc_ix(d, 2);
+    double v0 = tv[0][p.l1 * NTAB + i0], v1 = tv[1][p.l2 * NTAB + i1], v2 = tv[2][p.l3 * NTAB + i2];
+    double d0 = td[0][p.l1 * NTAB + i0], d1 = td[1][p.l2 * NTAB + i1], d2 = td[2][p.l3 * NTAB + i2];
+    double v12 = v1 * v2;
+    double gr[3] = {d0 * v12, v0 * d1 * v2, v0 * v1 * d2};
+    double ph = v0 * v12;
+    if (SPACE == 0) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) out[c] = gr[0] * gp[c] + gr[1] * gp[3 + c] + gr[2] * gp[6 + c];
+        return ph * gp[9];
+    } else {
+        int cp = desc_comp(d);
+        double sg = desc_neg(d) ? -1.0 : 1.0;
+        double val = sg * ph;
+        double cu[3];
+        cu[cp] = 0.0;
+        cu[(cp + 1) % 3] = sg * gr[(cp + 2) % 3];
+        cu[(cp + 2) % 3] = -sg * gr[(cp + 1) % 3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            out[c] = gp[3 * c] * cu[0] + gp[3 * c + 1] * cu[1] + gp[3 * c + 2] * cu[2];
+            out[3 + c] = val * gp[9 + 3 * cp + c];
+        }
+        return val * gp[18 + cp];
+    }
+}
+
 // load-vector contribution of row d at one point
 template <int SPACE>
-__device__ inline double load_entry(unsigned d, Pt p, const double (*tv)[NTAB * NPT], const double *gp)
+__device__ __forceinline__ double load_entry(unsigned d, Pt p, const double (*tv)[NTAB * NPT], const double *gp)
 {
-    double v = tv[0][desc_ix(d, 0) * NPT + p.l1] * tv[1][desc_ix(d, 1) * NPT + p.l2] *
-               tv[2][desc_ix(d, 2) * NPT + p.l3];
+    double v = tv[0][p.l1 * NTAB + desc_ix(d, 0)] * tv[1][p.l2 * NTAB + desc_ix(d, 1)] *
+               tv[2][p.l3 * NTAB + desc_ix(d, 2)];
     if (SPACE == 0) return v * gp[9];
     return (desc_neg(d) ? -v : v) * gp[18 + desc_comp(d)];
 }
@@ -286,6 +322,16 @@ __device__ bool element_setup(const ElemArgs &a, int e, ElemShared &es)
     return es.ok != 0;
 }
 
+// Make this wave's LDS writes visible to its other lanes: wait for the LDS
+// counter only (s_waitcnt lgkmcnt(0), vmcnt/expcnt left at their maxima so
+// outstanding global stores are not waited for) and keep the compiler from
+// moving LDS accesses across.
+__device__ __forceinline__ void wave_sync_lds()
+{
+    __builtin_amdgcn_s_waitcnt(0xC07F);
+    __builtin_amdgcn_wave_barrier();
+}
+
 // write an accumulator tile (rows R.., cols C..) of a symmetric matrix into
 // column-major `out` (leading dimension ld, valid extent lim x lim), both the
 // tile and its mirror; `stage` is this wave's 16 x 17 LDS scratch.
@@ -301,9 +347,7 @@ __device__ inline void write_sym_tile(V acc, int R, int C, int lim, S *out, size
     if (R != C) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) stage[M::row(lane, j) * 17 + (lane & 15)] = (S)acc[j];
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        wave_sync_lds();
         int r2 = R + (lane & 15);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -311,18 +355,11 @@ __device__ inline void write_sym_tile(V acc, int R, int C, int lim, S *out, size
             S v = stage[(lane & 15) * 17 + 4 * (lane >> 4) + j];
             if (r2 < lim && c2 < lim) out[(size_t)c2 * ld + r2] = v;
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        wave_sync_lds();
     }
 }
 
-__device__ inline void wave_sync_lds()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
+
 
 // General element kernel: one workgroup per (element, 128 x 128 block of K).
 template <typename T, int SPACE>
@@ -436,38 +473,50 @@ __global__ __launch_bounds__(WG) void elem_kernel(ElemArgs a)
 }
 
 // Single-block element kernel (N <= 128): K accumulates in registers over the
-// whole quadrature, then (when the caller asks for it and nb <= NBF) the
-// bubble dofs are condensed in place of a separate pass:
-//   Abb = L L^T (FP64, LDS), Y = L^-1 [Abi | Bb], Aii -= Y^T Y, Bi -= Y^T y_b
-//   (FP64 MFMA on the register tiles), X = L^-T Y = [ASchur | BSchur]
-// -- the algebra of stc_fwd_herm (stc.F90:338-416) with Aib Abb^-1 Abi
-// written as (L^-1 Abi)^T (L^-1 Abi).
-constexpr int NBF = 32;     // largest bubble block condensed in the fused path
-constexpr int LDS1 = BR + 4; // LDS stride of the single-block kernel's A columns
+// whole quadrature; then, when the caller asks for it and nb <= NBF, the
+// bubble dofs are condensed in the same launch (the algebra of stc_fwd_herm /
+// stc_fwd_gen, stc.F90:338-509):
+//   Gaussian elimination without pivoting of [Abb | I] (Abb SPD: Abb =
+//   L_u D L_u^T, the pivots D > 0 are the SPD check) gives D and L_u^-1;
+//   then on FP64 MFMA: Z = L_u^-1 [Abi | Bb];  Aii -= Z^T D^-1 Z and
+//   Bi -= Z^T D^-1 z_b on the register tiles (Aib Abb^-1 Abi = Z^T D^-1 Z);
+//   X = L_u^-T D^-1 Z = Abb^-1 [Abi | Bb] = [ASchur | BSchur].
+// The A chunks are double buffered: the MFMAs of chunk c are issued before
+// the VALU/LDS work that builds chunk c+1, so the two overlap.
+constexpr int NBF = 32;      // largest bubble block condensed in the fused path
+constexpr int LDS1 = BR + 16; // LDS stride of the single-block kernel's A columns (see LDA)
 struct CondArgs {
     const long long *off_c, *off_s;
     double *aii, *bi, *aschur, *bschur;
 };
 
+template <typename T> struct SingleCfg {
+    static constexpr int KCS = sizeof(T) == 4 ? 48 : 24; // A columns per chunk
+};
+
 template <typename T, int SPACE>
-__global__ __launch_bounds__(WG) void elem_single_kernel(ElemArgs a, CondArgs cd)
+__global__ __launch_bounds__(WG, 2) void elem_single_kernel(ElemArgs a, CondArgs cd)
 {
     typedef Mfma<T> M;
     typedef typename M::acc_t acc_t;
     typedef Mfma<double> MD;
     constexpr int NC = SPACE == 0 ? 3 : 6;
-    constexpr int QC = KC / NC;
-    constexpr int GQ = 64;
+    constexpr int KCS = SingleCfg<T>::KCS;
+    constexpr int QC = KCS / NC;               // points per chunk
+    constexpr int GQ = 64;                     // points per geometry pass (multiple of QC)
     constexpr int GS = SPACE == 0 ? 10 : 21;
     constexpr int NSLOT = 9;
-    constexpr int AC_D = (KC * LDS1 * (int)sizeof(T) + 7) / 8;   // A chunk, in doubles
-    constexpr int KB_D = NBF * LDS1 + 2 * LDS1;                  // bubble rows + two vectors
-    constexpr int SM_D = AC_D > KB_D ? AC_D : KB_D;
+    constexpr int BUF = KCS * LDS1;            // one A buffer, in T
+    constexpr int AC_D = (2 * BUF * (int)sizeof(T) + 7) / 8;
+    constexpr int CW_D = NBF * LDS1 + NBF * 33 + NBF + LDS1 + 4 * 16 * 17; // output / condensation workspace
+    constexpr int SM_D = AC_D > CW_D ? AC_D : CW_D;
+    static_assert(GQ % QC == 0, "geometry passes hold whole chunks");
 
     __shared__ ElemShared es;
     __shared__ double tv[3][NTAB * NPT], td[3][NTAB * NPT];
     __shared__ unsigned dsp[BR];
     __shared__ unsigned dgeo[MAXN_H1];
+    __shared__ unsigned pix[MAXQ];
     __shared__ double geo[GQ * GS];
     __shared__ double smem[SM_D];
     __shared__ int s_fail;
@@ -483,13 +532,25 @@ __global__ __launch_bounds__(WG) void elem_single_kernel(ElemArgs a, CondArgs cd
     if (wave < 3) fill_tables(es.nq[wave], tv[wave], td[wave]);
     for (int r = tid; r < BR; r += WG) dsp[r] = r < n ? (SPACE == 0 ? desc_h1(r, es.o) : desc_hcurl(r, es.o)) : 0u;
     for (int k = tid; k < a.ngeo; k += WG) dgeo[k] = desc_h1(k, es.o);
+    for (int l = tid; l < nint; l += WG) {
+        Pt p = point_ix(l, es.nq);
+        pix[l] = (unsigned)p.l1 | ((unsigned)p.l2 << 4) | ((unsigned)p.l3 << 8);
+    }
     if (tid == 0) s_fail = 0;
     __syncthreads();
 
     const double *xn = a.xnod + 3 * (size_t)a.ngeo * e;
     const double coef = a.coef[e];
+    // this thread's row of A and its half of each chunk's points
+    const int row = tid & (BR - 1), half = tid >> 7;
+    const unsigned drow = dsp[row];
+    const bool row_ok = row < n;
     // slots: tile row w (tiles w..7) then tile row 7-w (tiles 7-w..7): 9 per wave
     const int tiA = wave, tiB = 7 - wave, nA = 8 - wave;
+    const int offA = tiA * 16 + (lane & 15), offB = tiB * 16 + (lane & 15);
+    int offS[NSLOT];
+#pragma unroll
+    for (int s = 0; s < NSLOT; ++s) offS[s] = (s < nA ? tiA + s : s - 1) * 16 + (lane & 15);
     acc_t acc[NSLOT];
 #pragma unroll
     for (int s = 0; s < NSLOT; ++s) acc[s] = acc_t{0, 0, 0, 0};
@@ -501,68 +562,84 @@ __global__ __launch_bounds__(WG) void elem_single_kernel(ElemArgs a, CondArgs cd
     }
     double bsum = 0.0;
 
+    auto build = [&](int g0, int l0, int gend, T *buf) {
+#pragma unroll
+        for (int qq = 0; qq < QC / 2; ++qq) {
+            const int q = 2 * qq + half, lq = l0 + q;
+            double out[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) out[c] = 0.0;
+            if (row_ok && lq < gend) {
+                unsigned pk = pix[g0 + lq];
+                Pt p{(int)(pk & 15), (int)((pk >> 4) & 15), (int)(pk >> 8)};
+                bsum += a_entries_load<SPACE>(drow, p, tv, td, geo + lq * GS, out);
+            }
+#pragma unroll
+            for (int c = 0; c < NC; ++c) buf[(NC * q + c) * LDS1 + row] = (T)out[c];
+        }
+    };
+    auto mfma = [&](const T *buf) {
+#pragma unroll
+        for (int k0 = 0; k0 < KCS; k0 += 4) {
+            const T *col = buf + (k0 + (lane >> 4)) * LDS1;
+            const T aA = col[offA], aB = col[offB];
+#pragma unroll
+            for (int s = 0; s < NSLOT; ++s) acc[s] = M::mma(s < nA ? aA : aB, col[offS[s]], acc[s]);
+        }
+    };
+
     for (int g0 = 0; g0 < nint; g0 += GQ) {
-        for (int lq = tid; lq < GQ && g0 + lq < nint; lq += WG)
-            if (point_geometry<SPACE>(es, g0 + lq, tv, td, dgeo, xn, a.ngeo, coef, a.load, geo + lq * GS))
-                es.negjac = 1;
-        __syncthreads();
         const int gend = min(GQ, nint - g0);
-        if (tid < n)
-            for (int lq = 0; lq < gend; ++lq)
-                bsum += load_entry<SPACE>(dsp[tid], point_ix(g0 + lq, es.nq), tv, geo + lq * GS);
-        for (int l0 = 0; l0 < gend; l0 += QC) {
-            for (int it = tid; it < BR * QC; it += WG) {
-                int r = it % BR, q = it / BR, lq = l0 + q;
-                double out[NC];
-#pragma unroll
-                for (int c = 0; c < NC; ++c) out[c] = 0.0;
-                if (r < n && lq < gend)
-                    a_entries<SPACE>(dsp[r], point_ix(g0 + lq, es.nq), tv, td, geo + lq * GS, out);
-#pragma unroll
-                for (int c = 0; c < NC; ++c) ac[(NC * q + c) * LDS1 + r] = (T)out[c];
-            }
-            __syncthreads();
-#pragma unroll
-            for (int k0 = 0; k0 < KC; k0 += 4) {
-                const T *col = ac + (k0 + (lane >> 4)) * LDS1;
-                T aA = col[tiA * 16 + (lane & 15)], aB = col[tiB * 16 + (lane & 15)];
-                T b[8];
-#pragma unroll
-                for (int tj = 0; tj < 8; ++tj) b[tj] = col[tj * 16 + (lane & 15)];
-#pragma unroll
-                for (int s = 0; s < NSLOT; ++s)
-                    if (valid & (1u << s)) acc[s] = M::mma(s < nA ? aA : aB, b[s < nA ? tiA + s : s - 1], acc[s]);
-            }
+        if (tid < gend && point_geometry<SPACE>(es, g0 + tid, tv, td, dgeo, xn, a.ngeo, coef, a.load,
+                                                 geo + tid * GS))
+            es.negjac = 1;
+        __syncthreads();
+        const int nch = (gend + QC - 1) / QC;
+        build(g0, 0, gend, ac);
+        __syncthreads();
+        for (int c = 0; c < nch; ++c) {
+            mfma(ac + (c & 1) * BUF);
+            if (c + 1 < nch) build(g0, (c + 1) * QC, gend, ac + ((c + 1) & 1) * BUF);
             __syncthreads();
         }
     }
 
-    if (a.bout && tid < n) a.bout[a.off_v[e] + tid] = bsum;
-    // element matrix (optional in the fused form)
-    if (a.kout) {
+    // ---- outputs: load vector, K, and the condensed system ----
+    // (the main loop ended on a barrier: the A buffers are free)
+    double *kb = smem;              // bubble rows b < nb: Abi cols 0..ni-1, Abb cols ni..n-1, Bb col n
+    double *wm = kb + NBF * LDS1;   // L_u^-1, leading dimension 33
+    double *rd = wm + NBF * 33;     // 1 / D(j)
+    double *bv = rd + NBF;          // load vector
+    double *stg = bv + LDS1;        // 4 x 16 x 17 wave stages
+    T *kstage = reinterpret_cast<T *>(stg + wave * (16 * 17));
+    if (half == 1) bv[row] = bsum;
+    __syncthreads();
+    if (half == 0) bsum += bv[row];
+    const int nb = es.nb, ni = n - nb;
+    const bool cond = cd.aii != nullptr && nb <= NBF;
+    __syncthreads();
+    if (half == 0 && row_ok) {
+        if (a.bout) a.bout[a.off_v[e] + row] = bsum;
+        bv[row] = bsum;
+        if (cond && row >= ni) kb[(row - ni) * LDS1 + n] = bsum;
+    }
+    auto write_k = [&]() {
         T *kout = (T *)a.kout + a.off_k[e];
         const size_t ld = a.ld_k > 0 ? (size_t)a.ld_k : (size_t)n;
-        T *stage = ac + wave * (16 * 17);
 #pragma unroll
         for (int s = 0; s < NSLOT; ++s) {
             if (!(valid & (1u << s))) continue;
             int ti = s < nA ? tiA : tiB, tj = s < nA ? tiA + s : s - 1;
-            write_sym_tile<acc_t, T, M>(acc[s], ti * 16, tj * 16, n, kout, ld, stage, lane);
+            write_sym_tile<acc_t, T, M>(acc[s], ti * 16, tj * 16, n, kout, ld, kstage, lane);
         }
-    }
-    const int nb = es.nb, ni = n - nb;
-    if (!cd.aii || nb > NBF) {
+    };
+    if (!cond) {
+        if (a.kout) write_k();
         if (tid == 0 && a.status) a.status[e] = es.negjac ? HP3D_NEG_JACOBIAN : 0;
         return;
     }
-    __syncthreads(); // the A-chunk area becomes the condensation workspace
-    double *kb = smem;                     // kb[b * LDS1 + c] = K(ni + b, c)
-    double *bv = smem + NBF * LDS1;        // load vector
-    double *yb = bv + LDS1;                // L^-1 Bb, then Abb^-1 Bb
-    double *aii = cd.aii + cd.off_c[e];
-    double *bi = cd.bi + a.off_v[e];
-    if (tid < n) bv[tid] = bsum;
-    // bubble rows of K from the register tiles
+    // bubble rows of K from the register tiles; L_u^-1 starts as the identity
+    for (int idx = tid; idx < NBF * 33; idx += WG) wm[idx] = (idx / 33 == idx % 33) ? 1.0 : 0.0;
 #pragma unroll
     for (int s = 0; s < NSLOT; ++s) {
         if (!(valid & (1u << s))) continue;
@@ -578,47 +655,80 @@ __global__ __launch_bounds__(WG) void elem_single_kernel(ElemArgs a, CondArgs cd
         }
     }
     __syncthreads();
-    if (nb > 0 && wave == 0) { // Cholesky of Abb (lower, in kb[i][ni + j]) by one wave
-        for (int j = 0; j < nb; ++j) {
-            double d = kb[j * LDS1 + ni + j];
-            if (!(d > 0.0)) {
-                if (lane == 0) s_fail = 1;
-                d = 1.0;
-            }
-            d = sqrt(d);
-            wave_sync_lds();
-            if (lane == 0) kb[j * LDS1 + ni + j] = d;
-            for (int i = j + 1 + lane; i < nb; i += 64) kb[i * LDS1 + ni + j] /= d;
-            wave_sync_lds();
-            int m = nb - j - 1;
-            for (int idx = lane; idx < m * m; idx += 64) {
-                int i = j + 1 + idx % m, k = j + 1 + idx / m;
-                if (i >= k) kb[i * LDS1 + ni + k] -= kb[i * LDS1 + ni + j] * kb[k * LDS1 + ni + j];
-            }
-            wave_sync_lds();
+    if (a.kout) write_k();
+    // Gaussian elimination without pivoting of [Abb | I] (Abb = L_u D L_u^T,
+    // SPD: every pivot D(j) > 0): the Abb block becomes U = D L_u^T and the
+    // identity block L_u^-1.  Items (row i, column c) of a step are spread
+    // over the workgroup, 32 columns per row.
+    double *li = wm; // L_u^-1, leading dimension 33 (unit lower; zero above)
+    for (int j = 0; j < nb; ++j) {
+        const double piv = kb[j * LDS1 + ni + j];
+        const double rp = piv > 0.0 ? 1.0 / piv : 0.0;
+        if (tid == 0) {
+            rd[j] = rp;
+            if (!(piv > 0.0)) s_fail = 1;
         }
-    }
-    __syncthreads();
-    // forward solve Y = L^-1 [Abi | Bb]: one column per thread
-    if (tid <= ni && nb > 0) {
-        const int c = tid;
-        for (int i = 0; i < nb; ++i) {
-            double s = c < ni ? kb[i * LDS1 + c] : bv[ni + i];
-            for (int k = 0; k < i; ++k) s -= kb[i * LDS1 + ni + k] * (c < ni ? kb[k * LDS1 + c] : yb[k]);
-            s /= kb[i * LDS1 + ni + i];
-            if (c < ni) kb[i * LDS1 + c] = s;
-            else yb[i] = s;
+        const int nit = (nb - j - 1) * NBF;
+        for (int it = tid; it < nit; it += WG) {
+            const int i = j + 1 + (it / NBF), c = it % NBF;
+            if (c >= nb) continue;
+            const double m = kb[i * LDS1 + ni + j] * rp;
+            if (c > j) kb[i * LDS1 + ni + c] -= m * kb[j * LDS1 + ni + c];
+            else li[i * 33 + c] -= m * li[j * 33 + c];
         }
+        __syncthreads();
     }
+
+    // GEMMs against the right-hand block [Abi | Bb] on FP64 MFMA.  Output
+    // tiles: rows 16*tr.. (bubble index), columns 16*tc.. (ni Abi columns,
+    // then Bb).  Z = L_u^-1 R; X = U^-1 Z = L_u^-T (D^-1 Z).
+    const int ntr = (nb + 15) / 16, nct = (ni + 16) / 16, ntile = ntr * nct;
+    auto rhs_col = [&](int c) { return c < ni ? c : (c == ni ? n : -1); };
+    auto gemm_li = [&](bool second, d4_t *out) {
+#pragma unroll
+        for (int u = 0; u < 5; ++u) {
+            int t = wave + 4 * u;
+            out[u] = d4_t{0, 0, 0, 0};
+            if (t >= ntile) continue;
+            int tr = t / nct, tc = t % nct;
+            int ar = tr * 16 + (lane & 15), pc = rhs_col(tc * 16 + (lane & 15));
+            for (int k0 = 0; k0 < nb; k0 += 4) {
+                int k = k0 + (lane >> 4);
+                double av = 0.0, bw = 0.0;
+                if (ar < nb && k < nb) av = second ? li[k * 33 + ar] : li[ar * 33 + k];
+                if (k < nb && pc >= 0) bw = second ? kb[k * LDS1 + pc] * rd[k] : kb[k * LDS1 + pc];
+                out[u] = MD::mma(av, bw, out[u]);
+            }
+        }
+    };
+    auto store_rhs = [&](const d4_t *in) {
+#pragma unroll
+        for (int u = 0; u < 5; ++u) {
+            int t = wave + 4 * u;
+            if (t >= ntile) continue;
+            int tr = t / nct, tc = t % nct;
+            int pc = rhs_col(tc * 16 + (lane & 15));
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                int b = tr * 16 + MD::row(lane, j);
+                if (b < nb && pc >= 0) kb[b * LDS1 + pc] = in[u][j];
+            }
+        }
+    };
+    d4_t yt[5];
+    if (nb > 0) gemm_li(false, yt);
     __syncthreads();
-    // Bi = Bb_i - Y^T y_b
+    if (nb > 0) store_rhs(yt);
+    __syncthreads();
+    // Z now sits in the bubble rows.  Bi = b_i - Z^T D^-1 z_b
+    double *aii = cd.aii + cd.off_c[e];
+    double *bi = cd.bi + a.off_v[e];
     if (tid < ni) {
-        double s = bv[tid];
-        for (int k = 0; k < nb; ++k) s -= kb[k * LDS1 + tid] * yb[k];
-        bi[tid] = s;
+        double sacc = bv[tid];
+        for (int k = 0; k < nb; ++k) sacc -= kb[k * LDS1 + tid] * rd[k] * kb[k * LDS1 + n];
+        bi[tid] = sacc;
     }
-    // Aii = Kii - Y^T Y on the register tiles, in FP64
-    __shared__ double stage_d[4 * 16 * 17];
+    // Aii = Kii - Z^T D^-1 Z on the register tiles, in FP64
 #pragma unroll
     for (int s = 0; s < NSLOT; ++s) {
         if (!(valid & (1u << s))) continue;
@@ -632,13 +742,13 @@ __global__ __launch_bounds__(WG) void elem_single_kernel(ElemArgs a, CondArgs cd
         for (int k0 = 0; k0 < nb; k0 += 4) {
             int k = k0 + (lane >> 4);
             int ra = R + (lane & 15), cb = C + (lane & 15);
-            double av = (k < nb && ra < ni) ? kb[k * LDS1 + ra] : 0.0;
+            double av = (k < nb && ra < ni) ? kb[k * LDS1 + ra] * rd[k] : 0.0;
             double bw = (k < nb && cb < ni) ? kb[k * LDS1 + cb] : 0.0;
             up = MD::mma(av, bw, up);
         }
-        // up[j] is (Y^T Y)(R + MD::row(lane,j), C + lane%16); bring it to the
-        // accumulator's row order through this wave's stage
-        double *st = stage_d + wave * (16 * 17);
+        // up[j] is (Z^T D^-1 Z)(R + MD::row(lane,j), C + lane%16): bring it to
+        // the accumulator's row order through this wave's stage
+        double *st = stg + wave * (16 * 17);
 #pragma unroll
         for (int j = 0; j < 4; ++j) st[MD::row(lane, j) * 17 + (lane & 15)] = up[j];
         wave_sync_lds();
@@ -647,20 +757,15 @@ __global__ __launch_bounds__(WG) void elem_single_kernel(ElemArgs a, CondArgs cd
         wave_sync_lds();
         write_sym_tile<d4_t, double, M>(da, R, C, ni, aii, (size_t)ni, st, lane);
     }
+    // X = L_u^-T D^-1 Z = Abb^-1 [Abi | Bb]
+    if (nb > 0) gemm_li(true, yt);
     __syncthreads();
-    // X = L^-T Y -> ASchur (nb x ni, leading dimension nb), BSchur
-    if (tid <= ni && nb > 0) {
-        const int c = tid;
-        double *xs = c < ni ? cd.aschur + cd.off_s[e] + (size_t)c * nb : cd.bschur + a.off_v[e];
-        for (int i = nb - 1; i >= 0; --i) {
-            double s = c < ni ? kb[i * LDS1 + c] : yb[i];
-            for (int k = i + 1; k < nb; ++k) s -= kb[k * LDS1 + ni + i] * (c < ni ? kb[k * LDS1 + c] : yb[k]);
-            s /= kb[i * LDS1 + ni + i];
-            if (c < ni) kb[i * LDS1 + c] = s;
-            else yb[i] = s;
-            xs[i] = s;
-        }
-    }
+    if (nb > 0) store_rhs(yt);
+    __syncthreads();
+    // ASchur (nb x ni, leading dimension nb) and BSchur, coalesced
+    double *xs = cd.aschur + cd.off_s[e];
+    for (int idx = tid; idx < nb * ni; idx += WG) xs[idx] = kb[(idx % nb) * LDS1 + idx / nb];
+    if (tid < nb) cd.bschur[a.off_v[e] + tid] = kb[tid * LDS1 + n];
     if (tid == 0 && a.status) a.status[e] = s_fail ? HP3D_E_FACTOR : (es.negjac ? HP3D_NEG_JACOBIAN : 0);
 }
 
