@@ -48,11 +48,11 @@ def algorithmic_flops(batch):
     """k N (N+1) per element (SYRK count, k = 3Q for H1, 2 x 3Q for H(curl))."""
     from hp3d_amd.elem import nquad
 
-    q = np.prod(nquad(batch.norder_h, batch.nori_f.cpu().numpy()), axis=1)
+    q = np.prod(nquad(batch.norder_h, batch.nori_f_h), axis=1)
     k = 3 * q * (1 if batch.space == 0 else 2)
-    n = batch.n
+    n = batch._n
     fk = (k * n * (n + 1)).astype(np.float64)
-    nb, ni = batch.nb.astype(np.float64), batch.ni.astype(np.float64)
+    nb, ni = batch._nb.astype(np.float64), batch._ni.astype(np.float64)
     fc = nb ** 3 / 3 + 2 * nb ** 2 * (ni + 1) + 2 * ni ** 2 * nb + 2 * ni * nb
     return float(fk.sum()), float(fc.sum())
 
@@ -169,11 +169,21 @@ def main():
     s32, e32 = time_steps(batch, out32, hp3d_amd.FP32, args.steps, args.warmup)
     batch.check_status()
 
+    # the same FP32 step as one replay of a captured graph of the per-bin launches
+    import torch
+    g = batch.capture(out32, hp3d_amd.FP32)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        g.replay()
+    torch.cuda.synchronize()
+    sg = (time.perf_counter() - t0) / args.steps
+
     fk, fc = algorithmic_flops(batch)
     nel = batch.nel
     nb_total = float(batch.nb.sum())
-    kmax, kmed = rel_err_per_element(batch, out32["k"], out64["k"], batch.n * batch.n)
-    amax, amed = rel_err_per_element(batch, out32["aii"], out64["aii"], batch.ni * batch.ni)
+    kmax, kmed = rel_err_per_element(batch, out32["k"], out64["k"], batch._n * batch._n)
+    amax, amed = rel_err_per_element(batch, out32["aii"], out64["aii"], batch._ni * batch._ni)
 
     dropin = plain_dropin_rate(m, min(args.dropin_sample, nel)) if space == 0 else None
     value = nel / s32
@@ -197,7 +207,8 @@ def main():
         "interface_rows_per_s": float(batch.ni.sum()) / s32,
         "forms": {
             "batched_fp32": {"element_matrices_per_s": value, "ms_per_step": s32 * 1e3,
-                             "elem_kernel_ms": e32},
+                             "elem_kernel_ms": e32, "bins": [list(b) for b in batch.bins]},
+            "batched_fp32_graph": {"element_matrices_per_s": nel / sg, "ms_per_step": sg * 1e3},
             "batched_fp64": {"element_matrices_per_s": nel / s64, "ms_per_step": s64 * 1e3,
                              "elem_kernel_ms": e64,
                              "condensed_dof_per_s": nb_total / s64},

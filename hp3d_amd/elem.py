@@ -153,12 +153,21 @@ class ElementBatch:
     """Element data resident in device memory, ready for the batched kernels.
 
     norder (nel,19), nori_e (nel,12), nori_f (nel,6) int; xnod (nel,ngeo,3);
-    coef (nel,).  Outputs are packed: element e's N_e x N_e matrix starts at
-    off_k[e] with leading dimension N_e.
+    coef (nel,).  Elements are grouped into order bins (the replacement of the
+    OpenMP element loop by one launch per bin): bin 0 holds the elements whose
+    matrix fits one 128 x 128 block with at most 32 bubbles -- K stays in
+    registers and condensation runs in the same kernel -- and bin 1 the rest
+    (block-tiled element kernel, then the condensation kernel).  Internally the
+    elements are stored bin by bin; every accessor takes the caller's element
+    index.  Outputs are packed: element i's N_i x N_i matrix starts at
+    off_k[i] (internal order) with leading dimension N_i.
     """
 
+    FUSED_MAX_NDOF = 128
+    FUSED_MAX_NBUBBLE = 32
+
     def __init__(self, space, norder, nori_e, nori_f, xnod, coef, load=LOAD_SINSINSIN,
-                 device=None):
+                 device=None, binned=True):
         import torch
 
         _lib.init()
@@ -167,31 +176,48 @@ class ElementBatch:
         norder = np.ascontiguousarray(norder, dtype=np.int32)
         check_orders(norder)
         self.nel = norder.shape[0]
-        self.n, self.nb = [np.asarray(v, dtype=np.int64) for v in ndof(space, norder)]
-        self.ni = self.n - self.nb
-        self.max_ndof = int(self.n.max()) if self.nel else 1
-        self.max_nbubble = int(self.nb.max()) if self.nel else 0
-        self.norder_h = norder
+        n, nb = [np.asarray(v, dtype=np.int64) for v in ndof(space, norder)]
+        fused = (n <= self.FUSED_MAX_NDOF) & (nb <= self.FUSED_MAX_NBUBBLE)
+        if binned:
+            self.perm = np.concatenate([np.nonzero(fused)[0], np.nonzero(~fused)[0]])
+        else:
+            self.perm = np.arange(self.nel)
+        self.iperm = np.empty_like(self.perm)
+        self.iperm[self.perm] = np.arange(self.nel)
+        P = self.perm
+        # caller order (public) and internal (bin) order
+        self.n, self.nb, self.ni = n, nb, n - nb
+        self._n, self._nb = n[P], nb[P]
+        self._ni = self._n - self._nb
+        self.max_ndof = int(n.max()) if self.nel else 1
+        self.max_nbubble = int(nb.max()) if self.nel else 0
+        self.norder_h = norder[P]
+        nf = int(fused.sum()) if binned else 0
+        self.bins = []
+        for lo, hi in ((0, nf), (nf, self.nel)) if binned else ((0, self.nel),):
+            if hi > lo:
+                self.bins.append((lo, hi - lo, int(self._n[lo:hi].max()), int(self._nb[lo:hi].max())))
         dev = torch.device("cuda") if device is None else torch.device(device)
         self.device = dev
         t = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt).to(dev)
-        self.norder = t(norder, torch.int32)
-        self.nori_e = t(nori_e, torch.int32)
-        self.nori_f = t(nori_f, torch.int32)
-        xnod = np.ascontiguousarray(xnod, dtype=np.float64)
+        self.norder = t(self.norder_h, torch.int32)
+        self.nori_e = t(np.asarray(nori_e)[P], torch.int32)
+        self.nori_f = t(np.asarray(nori_f)[P], torch.int32)
+        self.nori_f_h = np.asarray(nori_f)[P]
+        xnod = np.ascontiguousarray(np.asarray(xnod, dtype=np.float64)[P])
         self.ngeo = xnod.shape[1]
         self.xnod = t(xnod, torch.float64)
-        self.coef = t(coef, torch.float64)
-        # packed offsets
-        n2 = self.n * self.n
+        self.coef = t(np.asarray(coef, dtype=np.float64)[P], torch.float64)
+        # packed offsets (internal order)
+        n2 = self._n * self._n
         self.off_k_h = np.concatenate([[0], np.cumsum(n2)[:-1]]).astype(np.int64)
-        self.off_v_h = np.concatenate([[0], np.cumsum(self.n)[:-1]]).astype(np.int64)
-        ni2 = self.ni * self.ni
+        self.off_v_h = np.concatenate([[0], np.cumsum(self._n)[:-1]]).astype(np.int64)
+        ni2 = self._ni * self._ni
         self.off_c_h = np.concatenate([[0], np.cumsum(ni2)[:-1]]).astype(np.int64)
-        nbi = self.nb * self.ni
+        nbi = self._nb * self._ni
         self.off_s_h = np.concatenate([[0], np.cumsum(nbi)[:-1]]).astype(np.int64)
         self.size_k = int(n2.sum())
-        self.size_v = int(self.n.sum())
+        self.size_v = int(self._n.sum())
         self.size_c = int(ni2.sum())
         self.size_s = int(nbi.sum())
         self.off_k = t(self.off_k_h, torch.int64)
@@ -199,8 +225,17 @@ class ElementBatch:
         self.off_c = t(self.off_c_h, torch.int64)
         self.off_s = t(self.off_s_h, torch.int64)
         self.status = torch.zeros(self.nel, dtype=torch.int32, device=dev)
-        ws = _lib.load().hp3d_gpu_stc_workspace(self.nel, self.max_nbubble)
+        lib = _lib.load()
+        ws = max([lib.hp3d_gpu_stc_workspace(cnt, mnb) for _, cnt, _, mnb in self.bins] + [8])
         self.workspace = torch.empty(max(int(ws) // 8, 1), dtype=torch.float64, device=dev)
+
+    def _ptrs(self, start):
+        """Device pointers of the per-element arrays from internal element `start`."""
+        def at(tensor, per):
+            if tensor is None:
+                return None
+            return ctypes.c_void_p(tensor.data_ptr() + start * per * tensor.element_size())
+        return at
 
     @classmethod
     def from_mesh(cls, space, mesh, load=None, device=None):
@@ -226,17 +261,19 @@ class ElementBatch:
         return out
 
     def element_matrices(self, out, precision=FP64, stream=None):
-        """Launch the element kernel into out['k'], out['b'] (no sync)."""
+        """Launch the element kernel(s) into out['k'], out['b'] (no sync)."""
         import torch
 
         lib = _lib.load()
         st = torch.cuda.current_stream().cuda_stream if stream is None else stream
-        rc = lib.hp3d_gpu_elem_batch(
-            self.space, precision, self.nel, _tptr(self.norder), _tptr(self.nori_e),
-            _tptr(self.nori_f), _tptr(self.xnod), self.ngeo, _tptr(self.coef), self.load,
-            _tptr(self.off_k), 0, _tptr(out["k"]), _tptr(self.off_v), _tptr(out["b"]),
-            _tptr(self.status), self.max_ndof, ctypes.c_void_p(st))
-        check(rc, "hp3d_gpu_elem_batch")
+        for lo, cnt, mnd, _mnb in self.bins:
+            at = self._ptrs(lo)
+            rc = lib.hp3d_gpu_elem_batch(
+                self.space, precision, cnt, at(self.norder, 19), at(self.nori_e, 12),
+                at(self.nori_f, 6), at(self.xnod, 3 * self.ngeo), self.ngeo, at(self.coef, 1),
+                self.load, at(self.off_k, 1), 0, _tptr(out["k"]), at(self.off_v, 1),
+                _tptr(out["b"]), at(self.status, 1), mnd, ctypes.c_void_p(st))
+            check(rc, "hp3d_gpu_elem_batch")
 
     def condense(self, out, precision=FP64, stream=None):
         """Launch static condensation of out['k'], out['b'] (no sync)."""
@@ -244,30 +281,49 @@ class ElementBatch:
 
         lib = _lib.load()
         st = torch.cuda.current_stream().cuda_stream if stream is None else stream
-        rc = lib.hp3d_gpu_stc_batch(
-            self.space, precision, self.nel, _tptr(self.norder), _tptr(self.off_k), 0,
-            _tptr(out["k"]), _tptr(self.off_v), _tptr(out["b"]), _tptr(self.off_c), 0,
-            _tptr(out["aii"]), _tptr(out["bi"]), _tptr(self.off_s), _tptr(out["aschur"]),
-            _tptr(out["bschur"]), _tptr(self.workspace), _tptr(self.status), self.max_nbubble,
-            ctypes.c_void_p(st))
-        check(rc, "hp3d_gpu_stc_batch")
+        for lo, cnt, _mnd, mnb in self.bins:
+            at = self._ptrs(lo)
+            rc = lib.hp3d_gpu_stc_batch(
+                self.space, precision, cnt, at(self.norder, 19), at(self.off_k, 1), 0,
+                _tptr(out["k"]), at(self.off_v, 1), _tptr(out["b"]), at(self.off_c, 1), 0,
+                _tptr(out["aii"]), _tptr(out["bi"]), at(self.off_s, 1), _tptr(out["aschur"]),
+                _tptr(out["bschur"]), _tptr(self.workspace), at(self.status, 1), mnb,
+                ctypes.c_void_p(st))
+            check(rc, "hp3d_gpu_stc_batch")
 
     def celem(self, out, precision=FP64, stream=None):
-        """Element matrices and condensation in one launch where the element
-        fits one block (fused), else two (no sync).  out['k'] / out['b'] may be
-        absent in the fused case."""
+        """Element matrices and condensation, one launch per bin where the
+        element fits one block (fused), else two (no sync).  out['k'] and
+        out['b'] may be absent when every bin is fused."""
         import torch
 
         lib = _lib.load()
         st = torch.cuda.current_stream().cuda_stream if stream is None else stream
-        rc = lib.hp3d_gpu_celem_batch(
-            self.space, precision, self.nel, _tptr(self.norder), _tptr(self.nori_e),
-            _tptr(self.nori_f), _tptr(self.xnod), self.ngeo, _tptr(self.coef), self.load,
-            _tptr(self.off_k), 0, _tptr(out.get("k")), _tptr(self.off_v), _tptr(out.get("b")),
-            _tptr(self.off_c), _tptr(out["aii"]), _tptr(out["bi"]), _tptr(self.off_s),
-            _tptr(out["aschur"]), _tptr(out["bschur"]), _tptr(self.workspace),
-            _tptr(self.status), self.max_ndof, self.max_nbubble, ctypes.c_void_p(st))
-        check(rc, "hp3d_gpu_celem_batch")
+        for lo, cnt, mnd, mnb in self.bins:
+            at = self._ptrs(lo)
+            rc = lib.hp3d_gpu_celem_batch(
+                self.space, precision, cnt, at(self.norder, 19), at(self.nori_e, 12),
+                at(self.nori_f, 6), at(self.xnod, 3 * self.ngeo), self.ngeo, at(self.coef, 1),
+                self.load, at(self.off_k, 1), 0, _tptr(out.get("k")), at(self.off_v, 1),
+                _tptr(out.get("b")), at(self.off_c, 1), _tptr(out["aii"]), _tptr(out["bi"]),
+                at(self.off_s, 1), _tptr(out["aschur"]), _tptr(out["bschur"]),
+                _tptr(self.workspace), at(self.status, 1), mnd, mnb, ctypes.c_void_p(st))
+            check(rc, "hp3d_gpu_celem_batch")
+
+    def capture(self, out, precision=FP64):
+        """The per-bin launches of celem() captured in a HIP graph (replay with
+        graph.replay()); inputs and outputs stay where they are."""
+        import torch
+
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            self.celem(out, precision)
+        torch.cuda.current_stream().wait_stream(s)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.celem(out, precision)
+        return g
 
     def run(self, precision=FP64, want_k=True, want_stc=True, out=None, fused=True):
         """Element matrices and (optionally) condensation; returns the outputs."""
@@ -282,7 +338,8 @@ class ElementBatch:
         return out
 
     def check_status(self):
-        s = self.status.cpu().numpy()
+        """Per-element status in the caller's order; raises on a fatal code."""
+        s = self.status.cpu().numpy()[self.iperm]
         if (s < 0).any():
             e = int(np.nonzero(s < 0)[0][0])
             check(int(s[e]), "element %d" % e)
@@ -290,21 +347,25 @@ class ElementBatch:
 
     # ---------------------------------------------------------- host views
     def k_of(self, out, e, key="k"):
-        n = int(self.n[e])
+        e = int(self.iperm[e])
+        n = int(self._n[e])
         o = int(self.off_k_h[e])
         return out[key][o:o + n * n].double().cpu().numpy().reshape(n, n).T
 
     def aii_of(self, out, e):
-        ni = int(self.ni[e])
+        e = int(self.iperm[e])
+        ni = int(self._ni[e])
         o = int(self.off_c_h[e])
         return out["aii"][o:o + ni * ni].cpu().numpy().reshape(ni, ni).T
 
     def aschur_of(self, out, e):
-        ni, nb = int(self.ni[e]), int(self.nb[e])
+        e = int(self.iperm[e])
+        ni, nb = int(self._ni[e]), int(self._nb[e])
         o = int(self.off_s_h[e])
         return out["aschur"][o:o + nb * ni].cpu().numpy().reshape(ni, nb).T
 
     def vec_of(self, out, key, e, length=None):
+        e = int(self.iperm[e])
         o = int(self.off_v_h[e])
-        n = int(self.n[e]) if length is None else int(length)
+        n = int(self._n[e]) if length is None else int(length)
         return out[key][o:o + n].cpu().numpy()
