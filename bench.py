@@ -145,6 +145,10 @@ def main():
     ap.add_argument("--workload", default="cfg1", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--dropin-sample", type=int, default=256)
+    ap.add_argument("--kernel-only", action="store_true",
+                    help="time only the FP32 element+condensation launches (for counter runs)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="run one device-to-device copy of MIB MiB and exit (counter scale)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: element work has no cross-GPU exchange here and "
@@ -152,6 +156,16 @@ def main():
         sys.exit(2)
 
     import torch
+
+    if args.copy_calibration:
+        n = args.copy_calibration * (1 << 20) // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n}))
+        return
 
     import hp3d_amd
     from hp3d_amd import mesh
@@ -161,6 +175,12 @@ def main():
     space = hp3d_amd.HCURL if args.workload == "cfg3" else hp3d_amd.H1
     batch = hp3d_amd.ElementBatch.from_mesh(space, m)
     out32 = batch.alloc_outputs(hp3d_amd.FP32)
+    if args.kernel_only:
+        s32, e32 = time_steps(batch, out32, hp3d_amd.FP32, args.steps, args.warmup)
+        batch.check_status()
+        print(json.dumps({"workload": args.workload, "ms_per_step": s32 * 1e3,
+                          "kernel_ms": e32, "bins": [list(b) for b in batch.bins]}))
+        return
     out64 = batch.alloc_outputs(hp3d_amd.FP64)
 
     # batched FP64 (reference form on the GPU) and FP32 (headline form)
@@ -170,7 +190,6 @@ def main():
     batch.check_status()
 
     # the same FP32 step as one replay of a captured graph of the per-bin launches
-    import torch
     g = batch.capture(out32, hp3d_amd.FP32)
     torch.cuda.synchronize()
     t0 = time.perf_counter()

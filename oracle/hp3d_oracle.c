@@ -6,7 +6,7 @@
  * product links or calls this file.
  *
  * Each function cites the reference routine it restates (paths relative to
- * the hp3d repository root).  How each one is pinned (tests/test_oracle_*.py):
+ * the hp3d repository root).  How each one is pinned (tests/test_oracle.py, tests/test_mesh.py):
  *
  *   orc_poly_legendre / orc_poly_ilegendre
  *       bitwise against the reference Polynomials.F90 compiled by

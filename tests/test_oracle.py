@@ -303,6 +303,7 @@ def test_hcurl_energy_identity_and_gradient_kernel():
     xg, wg = np.polynomial.legendre.leggauss(p + 1)
     xg, wg = 0.5 * (xg + 1), 0.5 * wg
     e = 0.0
+    bref = np.zeros(k.shape[0])
     for i, a in enumerate(xg):
         for j, bb in enumerate(xg):
             for l, cc in enumerate(xg):
@@ -313,7 +314,13 @@ def test_hcurl_energy_identity_and_gradient_kernel():
                 E = (s @ np.linalg.inv(jm)).T @ c            # covariant Piola
                 C = (jm @ cu.T) @ c / det                   # contravariant Piola
                 e += wg[i] * wg[j] * wg[l] * det * (C @ C + eps * E @ E)
+                # load J = (1 + 2 pi^2) sin(pi y) sin(pi z) e_x against every E_k
+                x = trilinear(xn, xi)
+                jx = (1 + 2 * np.pi ** 2) * np.sin(np.pi * x[1]) * np.sin(np.pi * x[2])
+                ek = s @ np.linalg.inv(jm)                  # (N, 3) physical E_k
+                bref += wg[i] * wg[j] * wg[l] * det * jx * ek[:, 0]
     assert abs(c @ k @ c - e) < 1e-12 * abs(e)
+    assert np.linalg.norm(b - bref) < 1e-12 * np.linalg.norm(bref)
     # curl part annihilates gradients: K_curl = K(eps=0) has grad(H1) in its kernel
     k0, _b, _ = oracle.elem_hcurl(no, ne, nf, xn, mu=1.0, eps=0.0)
     pts = rng.random((40, 3))
