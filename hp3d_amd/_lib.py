@@ -14,6 +14,8 @@ LIB_PATH = os.path.join(HERE, "libhp3d_gpu.so")
 H1, HCURL = 0, 1
 FP64, FP32 = 0, 1
 LOAD_ZERO, LOAD_SINSINSIN, LOAD_CONV, LOAD_ONE, LOAD_CURL_SIN = 0, 1, 2, 3, 10
+LOAD_POLY2 = 4
+FIELD_ERROR, FIELD_ADJOINT, FIELD_L2 = 0, 1, 2
 
 OK = 0
 NEG_JACOBIAN = 1
@@ -39,7 +41,11 @@ SIGNATURES = {
     "hp3d_gpu_stc_workspace": (_sz, [_i, _i]),
     "hp3d_gpu_celem_batch": (_i, [_i, _i, _i, _p, _p, _p, _p, _i, _p, _i, _p, _i, _p, _p, _p,
                                   _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _p]),
-    "hp3d_gpu_stc_bwd_batch": (_i, [_i, _i, _p, _p, _p, _p, _p, _p, _p, _p]),
+    "hp3d_gpu_stc_bwd_batch": (_i, [_i, _i, _p, _p, _p, _p, _p, _p, _p]),
+    "hp3d_gpu_field_batch": (_i, [_i, _i, _i, _p, _p, _p, _p, _i, _p, _p, _p, _p, _i, _p, _p,
+                                  _p, _p]),
+    "hp3d_gpu_gather_sum": (_i, [ctypes.c_longlong, _p, _p, _p, _p, _p]),
+    "hp3d_gpu_csr_spmv": (_i, [ctypes.c_longlong, _p, _p, _p, _p, _p, _p]),
 }
 
 

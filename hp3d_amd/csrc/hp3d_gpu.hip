@@ -145,6 +145,11 @@ __device__ inline void getf(int load, const double *x, double *f)
     case HP3D_LOAD_SINSINSIN: f[0] = 3.0 * pi * pi * sin(pi * x[0]) * sin(pi * x[1]) * sin(pi * x[2]); break;
     case HP3D_LOAD_CONV: f[0] = -2.0 * sin(pi * x[0]) * sinh(pi * x[1]); break;
     case HP3D_LOAD_ONE: f[0] = 1.0; break;
+    case HP3D_LOAD_POLY2: {
+        double u0 = x[0] * (1.0 - x[0]), u1 = x[1] * (1.0 - x[1]), u2 = x[2] * (1.0 - x[2]);
+        f[0] = 2.0 * (u1 * u2 + u0 * u2 + u0 * u1);
+        break;
+    }
     case HP3D_LOAD_CURL_SIN: f[0] = (1.0 + 2.0 * pi * pi) * sin(pi * x[1]) * sin(pi * x[2]); break;
     default: break;
     }
@@ -160,16 +165,18 @@ __device__ inline void block_ij(int b, int &I, int &J)
     I = b - J * (J + 1) / 2;
 }
 
-// geometry of one quadrature point (FP64): x, J, det J, J^-1 folded with the
-// weights into GS doubles at gp (geom3D.F90:77-91, geom.F90:57-113,
-// elem_opt.F90:101-122).  Returns true when det J < 0.
-template <int SPACE>
-__device__ __forceinline__ bool point_geometry(const ElemShared &es, int l, const double (*tv)[NTAB * NPT],
-                               const double (*td)[NTAB * NPT], const unsigned *dgeo,
-                               const double *xn, int ngeo, double coef, int load, double *gp)
+// the trilinear / isoparametric map at point p of the tensor rule with nq
+// points per direction (geom3D.F90:77-91, geom.F90:57-113): x, J (jm[c][i] =
+// d x_c / d xi_i), det J, inv[i][j] = d xi_i / d x_j and the Gauss weight
+__device__ __forceinline__ void point_map(Pt p, const int *nq, const double (*tv)[NTAB * NPT],
+                                          const double (*td)[NTAB * NPT], const unsigned *dgeo,
+                                          const double *xn, int ngeo, double x[3], double jm[3][3],
+                                          double inv[3][3], double &det, double &w)
 {
-    Pt p = point_ix(l, es.nq);
-    double x[3] = {0, 0, 0}, jm[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int c = 0; c < 3; ++c) {
+        x[c] = 0.0;
+        jm[c][0] = jm[c][1] = jm[c][2] = 0.0;
+    }
     for (int k = 0; k < ngeo; ++k) {
         unsigned d = dgeo[k];
         int i0 = desc_ix(d, 0), i1 = desc_ix(d, 1), i2 = desc_ix(d, 2);
@@ -186,10 +193,9 @@ __device__ __forceinline__ bool point_geometry(const ElemShared &es, int l, cons
             jm[c][2] += xc * g2;
         }
     }
-    double det = jm[0][0] * jm[1][1] * jm[2][2] + jm[1][0] * jm[2][1] * jm[0][2] +
-                 jm[2][0] * jm[0][1] * jm[1][2] - jm[2][0] * jm[1][1] * jm[0][2] -
-                 jm[0][0] * jm[2][1] * jm[1][2] - jm[1][0] * jm[0][1] * jm[2][2];
-    double inv[3][3]; // inv[i][j] = d xi_i / d x_j
+    det = jm[0][0] * jm[1][1] * jm[2][2] + jm[1][0] * jm[2][1] * jm[0][2] +
+          jm[2][0] * jm[0][1] * jm[1][2] - jm[2][0] * jm[1][1] * jm[0][2] -
+          jm[0][0] * jm[2][1] * jm[1][2] - jm[1][0] * jm[0][1] * jm[2][2];
     inv[0][0] = (jm[1][1] * jm[2][2] - jm[2][1] * jm[1][2]) / det;
     inv[1][0] = (-jm[1][0] * jm[2][2] + jm[2][0] * jm[1][2]) / det;
     inv[2][0] = (jm[1][0] * jm[2][1] - jm[2][0] * jm[1][1]) / det;
@@ -199,8 +205,19 @@ __device__ __forceinline__ bool point_geometry(const ElemShared &es, int l, cons
     inv[0][2] = (jm[0][1] * jm[1][2] - jm[1][1] * jm[0][2]) / det;
     inv[1][2] = (-jm[0][0] * jm[1][2] + jm[1][0] * jm[0][2]) / det;
     inv[2][2] = (jm[0][0] * jm[1][1] - jm[1][0] * jm[0][1]) / det;
-    double w = (0.5 * c_gw[es.nq[0] - 1][p.l1]) * (0.5 * c_gw[es.nq[1] - 1][p.l2]) *
-               (0.5 * c_gw[es.nq[2] - 1][p.l3]);
+    w = (0.5 * c_gw[nq[0] - 1][p.l1]) * (0.5 * c_gw[nq[1] - 1][p.l2]) * (0.5 * c_gw[nq[2] - 1][p.l3]);
+}
+
+// geometry of one quadrature point (FP64): x, J, det J, J^-1 folded with the
+// weights into GS doubles at gp (geom3D.F90:77-91, geom.F90:57-113,
+// elem_opt.F90:101-122).  Returns true when det J < 0.
+template <int SPACE>
+__device__ __forceinline__ bool point_geometry(const ElemShared &es, int l, const double (*tv)[NTAB * NPT],
+                               const double (*td)[NTAB * NPT], const unsigned *dgeo,
+                               const double *xn, int ngeo, double coef, int load, double *gp)
+{
+    double x[3], jm[3][3], inv[3][3], det, w;
+    point_map(point_ix(l, es.nq), es.nq, tv, td, dgeo, xn, ngeo, x, jm, inv, det, w);
     double wd = det * w;
     double f[3];
     getf(load, x, f);
@@ -898,8 +915,8 @@ struct BwdArgs {
     int space, nel;
     const int *norder;
     const long long *off_s, *off_v;
-    const double *aschur, *bschur, *xi;
-    double *xb;
+    const double *aschur, *bschur;
+    double *x;
 };
 
 __global__ __launch_bounds__(WG) void stc_bwd_kernel(BwdArgs a)
@@ -913,12 +930,326 @@ __global__ __launch_bounds__(WG) void stc_bwd_kernel(BwdArgs a)
     dof_counts(a.space, o, n, nb);
     int ni = n - nb;
     const double *A = a.aschur + a.off_s[e];
-    const double *xi = a.xi + a.off_v[e];
+    double *x = a.x + a.off_v[e];
     for (int q = threadIdx.x; q < nb; q += WG) {
         double s = 0.0;
-        for (int i = 0; i < ni; ++i) s += A[(size_t)i * nb + q] * xi[i];
-        a.xb[a.off_v[e] + q] = a.bschur[a.off_v[e] + q] - s;
+        for (int i = 0; i < ni; ++i) s += A[(size_t)i * nb + q] * x[i];
+        x[ni + q] = a.bschur[a.off_v[e] + q] - s;
     }
+}
+
+// ------------------------------------------------------- field evaluation
+//
+// One workgroup per element evaluates finite-element fields from element dof
+// vectors at the quadrature points (soleval, element/util/soleval.F90, with
+// the dofs in element order) and integrates one of:
+//   HP3D_FIELD_ERROR    element_error (element/util/compute_error.F90:226-400):
+//                       set_3Dint rule with INTEGRATION = 2; out[2e] = errorH
+//                       (int |grad(u - u_h)|^2 + |u - u_h|^2) or errorE (curl
+//                       and value), out[2e+1] = the same norm of the exact u
+//   HP3D_FIELD_ADJOINT  out[e] = -coef_e int grad v_h . grad u_h (H1) or
+//                       -coef_e int v_h . u_h (H(curl)): lambda_e^T (dK_e /
+//                       dlog coef_e) u_e with a minus sign, i.e. dJ/dtheta_e
+//                       for coef = exp(theta), recomputed at the points of the
+//                       element rule without forming K_e
+//   HP3D_FIELD_L2       out[e] = int u_h . u_h and r[off_v[e] + k] = int
+//                       phi_k . u_h (the element mass matrix times u_e)
+// Threads own quadrature points; the dof loop reads descriptors and
+// coefficients from LDS (same address across the wave: broadcast).
+
+struct FieldArgs {
+    int space, mode, nel, ngeo, exact;
+    const int *norder, *nori_e, *nori_f;
+    const double *xnod, *coef;
+    const long long *off_v;
+    const double *u, *v;
+    double *out, *r;
+    int *status;
+};
+
+// exact solutions of the manufactured problems (value, gradient); the
+// selector is the load selector of the same problem
+__device__ inline void exact_h1(int sel, const double *x, double &u, double *du)
+{
+    const double pi = 3.14159265358979323846;
+    u = du[0] = du[1] = du[2] = 0.0;
+    if (sel == HP3D_LOAD_SINSINSIN) {
+        double sx = sin(pi * x[0]), sy = sin(pi * x[1]), sz = sin(pi * x[2]);
+        u = sx * sy * sz;
+        du[0] = pi * cos(pi * x[0]) * sy * sz;
+        du[1] = pi * sx * cos(pi * x[1]) * sz;
+        du[2] = pi * sx * sy * cos(pi * x[2]);
+    } else if (sel == HP3D_LOAD_CONV) {
+        double sx = sin(pi * x[0]), sh = sinh(pi * x[1]), z2 = x[2] * x[2];
+        u = sx * sh * z2;
+        du[0] = pi * cos(pi * x[0]) * sh * z2;
+        du[1] = pi * sx * cosh(pi * x[1]) * z2;
+        du[2] = 2.0 * sx * sh * x[2];
+    } else if (sel == HP3D_LOAD_POLY2) {
+        double a0 = x[0] * (1.0 - x[0]), a1 = x[1] * (1.0 - x[1]), a2 = x[2] * (1.0 - x[2]);
+        u = a0 * a1 * a2;
+        du[0] = (1.0 - 2.0 * x[0]) * a1 * a2;
+        du[1] = a0 * (1.0 - 2.0 * x[1]) * a2;
+        du[2] = a0 * a1 * (1.0 - 2.0 * x[2]);
+    }
+}
+
+__device__ inline void exact_hcurl(int sel, const double *x, double *e, double *curl)
+{
+    const double pi = 3.14159265358979323846;
+    for (int i = 0; i < 3; ++i) e[i] = curl[i] = 0.0;
+    if (sel == HP3D_LOAD_CURL_SIN) {
+        e[0] = sin(pi * x[1]) * sin(pi * x[2]);
+        curl[1] = pi * sin(pi * x[1]) * cos(pi * x[2]);
+        curl[2] = -pi * cos(pi * x[1]) * sin(pi * x[2]);
+    }
+}
+
+// set_3Dint brick rule (set_3D_int.F90:8-21,203-244): face orders as coded
+// (no swap), orders raised by INTEGRATION and capped at MAXP
+__device__ inline void quad_counts_plain(const ElemOrders &o, int integration, int nq[3])
+{
+    int m[3] = {o.b[0], o.b[1], o.b[2]};
+    for (int e = 0; e < 12; ++e) {
+        int d = edge_t(e).dir;
+        m[d] = o.pe[e] > m[d] ? o.pe[e] : m[d];
+    }
+    for (int f = 0; f < 6; ++f) {
+        FaceT ft = face_t(f);
+        m[ft.ax0] = o.fh[f] > m[ft.ax0] ? o.fh[f] : m[ft.ax0];
+        m[ft.ax1] = o.fv[f] > m[ft.ax1] ? o.fv[f] : m[ft.ax1];
+    }
+    for (int d = 0; d < 3; ++d) {
+        int q = m[d] + integration;
+        nq[d] = (q < MAXP ? q : MAXP) + 1;
+    }
+}
+
+// reference-coordinate value and gradient of the scalar dof d at point p
+__device__ __forceinline__ void dof_ref(unsigned d, Pt p, const double (*tv)[NTAB * NPT],
+                                        const double (*td)[NTAB * NPT], double &ph, double gr[3])
+{
+    int i0 = desc_ix(d, 0), i1 = desc_ix(d, 1), i2 = desc_ix(d, 2);
+    double v0 = tv[0][p.l1 * NTAB + i0], v1 = tv[1][p.l2 * NTAB + i1], v2 = tv[2][p.l3 * NTAB + i2];
+    double d0 = td[0][p.l1 * NTAB + i0], d1 = td[1][p.l2 * NTAB + i1], d2 = td[2][p.l3 * NTAB + i2];
+    ph = v0 * v1 * v2;
+    gr[0] = d0 * v1 * v2;
+    gr[1] = v0 * d1 * v2;
+    gr[2] = v0 * v1 * d2;
+}
+
+// field of coefficients c at point p in reference form: H1 value f[0] and
+// gradient g; H(curl) E_ref in f[0..2] and curl_ref in g
+template <int SPACE>
+__device__ __forceinline__ void eval_ref(int n, const unsigned *dsc, const double *c, Pt p,
+                                         const double (*tv)[NTAB * NPT], const double (*td)[NTAB * NPT],
+                                         double f[3], double g[3])
+{
+    for (int i = 0; i < 3; ++i) f[i] = g[i] = 0.0;
+    for (int k = 0; k < n; ++k) {
+        unsigned d = dsc[k];
+        double ph, gr[3];
+        dof_ref(d, p, tv, td, ph, gr);
+        double ck = c[k];
+        if (SPACE == 0) {
+            f[0] += ck * ph;
+            for (int i = 0; i < 3; ++i) g[i] += ck * gr[i];
+        } else {
+            int cp = desc_comp(d);
+            double sg = desc_neg(d) ? -ck : ck;
+            // E = s phi e_cp, curl E = s grad(phi) x e_cp
+            f[cp] += sg * ph;
+            g[(cp + 1) % 3] += sg * gr[(cp + 2) % 3];
+            g[(cp + 2) % 3] -= sg * gr[(cp + 1) % 3];
+        }
+    }
+}
+
+// physical value / gradient (H1) or E / curl E (H(curl)) from the reference form
+template <int SPACE>
+__device__ __forceinline__ void to_phys(const double f[3], const double g[3], double jm[3][3], double inv[3][3],
+                                        double det, double fp[3], double gp[3])
+{
+    if (SPACE == 0) {
+        fp[0] = f[0];
+        fp[1] = fp[2] = 0.0;
+        for (int j = 0; j < 3; ++j) gp[j] = g[0] * inv[0][j] + g[1] * inv[1][j] + g[2] * inv[2][j];
+    } else {
+        for (int j = 0; j < 3; ++j) {
+            fp[j] = f[0] * inv[0][j] + f[1] * inv[1][j] + f[2] * inv[2][j];
+            gp[j] = (jm[j][0] * g[0] + jm[j][1] * g[1] + jm[j][2] * g[2]) / det;
+        }
+    }
+}
+
+template <int SPACE>
+__global__ __launch_bounds__(WG) void field_kernel(FieldArgs a)
+{
+    constexpr int MAXN = SPACE == 0 ? MAXN_H1 : MAXN_E;
+    __shared__ ElemShared es;
+    __shared__ double tv[3][NTAB * NPT], td[3][NTAB * NPT];
+    __shared__ unsigned dsc[MAXN];
+    __shared__ unsigned dgeo[MAXN_H1];
+    __shared__ double cu[MAXN], cv[MAXN];
+    __shared__ double pw[3 * MAXQ];
+    __shared__ double red[2][WG / 64];
+
+    const int e = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) {
+        ElemShared s;
+        s.ok = decode_orders(a.norder + 19 * (size_t)e, a.nori_e + 12 * (size_t)e,
+                             a.nori_f + 6 * (size_t)e, s.o);
+        s.n = s.nb = s.nint = s.nh = 0;
+        s.nq[0] = s.nq[1] = s.nq[2] = 1;
+        if (s.ok) {
+            dof_counts(SPACE, s.o, s.n, s.nb);
+            int nbh;
+            dof_counts(0, s.o, s.nh, nbh);
+            if (a.mode == HP3D_FIELD_ERROR) quad_counts_plain(s.o, 2, s.nq);
+            else quad_counts(s.o, s.nq);
+            s.nint = s.nq[0] * s.nq[1] * s.nq[2];
+            if (a.ngeo > s.nh) s.ok = 0;
+        }
+        s.negjac = 0;
+        es = s;
+    }
+    __syncthreads();
+    if (!es.ok) {
+        if (tid == 0 && a.status) a.status[e] = HP3D_E_ORDER;
+        return;
+    }
+    const int n = es.n, nint = es.nint;
+    if (wave < 3) fill_tables(es.nq[wave], tv[wave], td[wave]);
+    const double *ue = a.u + a.off_v[e];
+    const double *ve = a.mode == HP3D_FIELD_ADJOINT ? a.v + a.off_v[e] : nullptr;
+    for (int k = tid; k < n; k += WG) {
+        dsc[k] = SPACE == 0 ? desc_h1(k, es.o) : desc_hcurl(k, es.o);
+        cu[k] = ue[k];
+        if (ve) cv[k] = ve[k];
+    }
+    for (int k = tid; k < a.ngeo; k += WG) dgeo[k] = desc_h1(k, es.o);
+    __syncthreads();
+
+    const double *xn = a.xnod + 3 * (size_t)a.ngeo * e;
+    const double coef = a.coef ? a.coef[e] : 1.0;
+    double acc0 = 0.0, acc1 = 0.0;
+    bool neg = false;
+    for (int l = tid; l < nint; l += WG) {
+        Pt p = point_ix(l, es.nq);
+        double x[3], jm[3][3], inv[3][3], det, w;
+        point_map(p, es.nq, tv, td, dgeo, xn, a.ngeo, x, jm, inv, det, w);
+        neg = neg || det < 0.0;
+        const double wd = w * det;
+        double f[3], g[3], fu[3], gu[3];
+        eval_ref<SPACE>(n, dsc, cu, p, tv, td, f, g);
+        to_phys<SPACE>(f, g, jm, inv, det, fu, gu);
+        if (a.mode == HP3D_FIELD_ERROR) {
+            double fe[3], ge[3];
+            if (SPACE == 0) {
+                exact_h1(a.exact, x, fe[0], ge);
+                fe[1] = fe[2] = 0.0;
+            } else {
+                exact_hcurl(a.exact, x, fe, ge);
+            }
+            for (int j = 0; j < 3; ++j) {
+                acc0 += ((ge[j] - gu[j]) * (ge[j] - gu[j]) + (fe[j] - fu[j]) * (fe[j] - fu[j])) * wd;
+                acc1 += (ge[j] * ge[j] + fe[j] * fe[j]) * wd;
+            }
+        } else if (a.mode == HP3D_FIELD_ADJOINT) {
+            double fv[3], gv[3];
+            eval_ref<SPACE>(n, dsc, cv, p, tv, td, f, g);
+            to_phys<SPACE>(f, g, jm, inv, det, fv, gv);
+            if (SPACE == 0) acc0 -= coef * (gu[0] * gv[0] + gu[1] * gv[1] + gu[2] * gv[2]) * wd;
+            else acc0 -= coef * (fu[0] * fv[0] + fu[1] * fv[1] + fu[2] * fv[2]) * wd;
+        } else {
+            acc0 += (fu[0] * fu[0] + fu[1] * fu[1] + fu[2] * fu[2]) * wd;
+            // weighted field pulled back to reference components: phi_k . u_h w det
+            if (SPACE == 0) {
+                pw[l] = fu[0] * wd;
+            } else {
+                for (int i = 0; i < 3; ++i)
+                    pw[3 * l + i] = (inv[i][0] * fu[0] + inv[i][1] * fu[1] + inv[i][2] * fu[2]) * wd;
+            }
+        }
+    }
+    if (neg) es.negjac = 1;
+    // workgroup sums
+    for (int off = 32; off > 0; off >>= 1) {
+        acc0 += __shfl_down(acc0, off);
+        acc1 += __shfl_down(acc1, off);
+    }
+    if (lane == 0) {
+        red[0][wave] = acc0;
+        red[1][wave] = acc1;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double s0 = 0.0, s1 = 0.0;
+        for (int w = 0; w < WG / 64; ++w) {
+            s0 += red[0][w];
+            s1 += red[1][w];
+        }
+        if (a.mode == HP3D_FIELD_ERROR) {
+            a.out[2 * (size_t)e] = s0;
+            a.out[2 * (size_t)e + 1] = s1;
+        } else {
+            a.out[e] = s0;
+        }
+        if (a.status) a.status[e] = es.negjac ? HP3D_NEG_JACOBIAN : 0;
+    }
+    if (a.mode == HP3D_FIELD_L2 && a.r) {
+        double *re = a.r + a.off_v[e];
+        for (int k = tid; k < n; k += WG) {
+            unsigned d = dsc[k];
+            double s = 0.0;
+            for (int l = 0; l < nint; ++l) {
+                Pt p = point_ix(l, es.nq);
+                double ph, gr[3];
+                dof_ref(d, p, tv, td, ph, gr);
+                if (SPACE == 0) s += ph * pw[l];
+                else s += (desc_neg(d) ? -ph : ph) * pw[3 * l + desc_comp(d)];
+            }
+            re[k] = s;
+        }
+    }
+}
+
+// Segmented gather-sum: out[t] = sum over j in [ptr[t], ptr[t+1]) of in[src[j]].
+// With src listing, for every nonzero of a global matrix (or entry of a global
+// vector), the positions of its element contributions in the packed condensed
+// matrices (vectors), this is the assembly of the element systems into CSR
+// values (mumps_sc.F90:266-323 accumulates the same triplets, duplicates
+// summed); the fixed summation order makes it deterministic.
+__global__ __launch_bounds__(WG) void gather_sum_kernel(long long nout, const long long *ptr,
+                                                       const long long *src, const double *in,
+                                                       double *out)
+{
+    long long t = (long long)blockIdx.x * WG + threadIdx.x;
+    if (t >= nout) return;
+    double s = 0.0;
+    for (long long j = ptr[t]; j < ptr[t + 1]; ++j) s += in[src[j]];
+    out[t] = s;
+}
+
+// y = A x for a CSR matrix (FP64 values, int32 column indices): 32 lanes per
+// row, each lane striding over the row's nonzeros (coalesced reads of
+// indices and values), then a 32-lane shuffle reduction.  HBM bound: 12
+// bytes per nonzero plus the gathered x.  The global condensed systems have
+// tens to hundreds of nonzeros per row.
+constexpr int SPMV_LPR = 32;
+__global__ __launch_bounds__(WG) void csr_spmv_kernel(long long nrows, const long long *indptr,
+                                                     const int *indices, const double *vals,
+                                                     const double *x, double *y)
+{
+    long long row = (long long)blockIdx.x * (WG / SPMV_LPR) + threadIdx.x / SPMV_LPR;
+    const int sub = threadIdx.x % SPMV_LPR;
+    double s = 0.0;
+    if (row < nrows) {
+        for (long long j = indptr[row] + sub; j < indptr[row + 1]; j += SPMV_LPR)
+            s += vals[j] * x[indices[j]];
+    }
+    for (int off = SPMV_LPR / 2; off > 0; off >>= 1) s += __shfl_down(s, off, SPMV_LPR);
+    if (row < nrows && sub == 0) y[row] = s;
 }
 
 // ------------------------------------------------------------ host state
@@ -1210,13 +1541,62 @@ int hp3d_gpu_celem_batch(int space, int precision, int nel, const int *norder, c
 
 int hp3d_gpu_stc_bwd_batch(int space, int nel, const int *norder, const long long *off_s,
                            const double *aschur, const long long *off_v, const double *bschur,
-                           const double *xi, double *xb, void *stream)
+                           double *x, void *stream)
 {
     if ((space != HP3D_H1 && space != HP3D_HCURL) || nel < 0) return HP3D_E_ARG;
     if (nel == 0) return 0;
-    if (!norder || !off_s || !aschur || !off_v || !bschur || !xi || !xb) return HP3D_E_ARG;
-    BwdArgs a{space, nel, norder, off_s, off_v, aschur, bschur, xi, xb};
+    if (!norder || !off_s || !aschur || !off_v || !bschur || !x) return HP3D_E_ARG;
+    BwdArgs a{space, nel, norder, off_s, off_v, aschur, bschur, x};
     hipLaunchKernelGGL(stc_bwd_kernel, dim3(nel), dim3(WG), 0, (hipStream_t)stream, a);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int hp3d_gpu_field_batch(int space, int mode, int nel, const int *norder, const int *nori_e,
+                         const int *nori_f, const double *xnod, int ngeo, const double *coef,
+                         const long long *off_v, const double *u, const double *v, int exact,
+                         double *out, double *r, int *status, void *stream)
+{
+    if ((space != HP3D_H1 && space != HP3D_HCURL) || nel < 0 || ngeo < 1 || ngeo > MAXN_H1)
+        return HP3D_E_ARG;
+    if (mode != HP3D_FIELD_ERROR && mode != HP3D_FIELD_ADJOINT && mode != HP3D_FIELD_L2)
+        return HP3D_E_ARG;
+    if (nel == 0) return 0;
+    if (!norder || !nori_e || !nori_f || !xnod || !off_v || !u || !out) return HP3D_E_ARG;
+    if (mode == HP3D_FIELD_ADJOINT && (!v || !coef)) return HP3D_E_ARG;
+    FieldArgs a{space, mode, nel, ngeo, exact, norder, nori_e, nori_f, xnod, coef, off_v, u, v,
+                out, r, status};
+    hipStream_t st = (hipStream_t)stream;
+    if (space == HP3D_H1) hipLaunchKernelGGL(field_kernel<0>, dim3(nel), dim3(WG), 0, st, a);
+    else hipLaunchKernelGGL(field_kernel<1>, dim3(nel), dim3(WG), 0, st, a);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int hp3d_gpu_gather_sum(long long nout, const long long *ptr, const long long *src,
+                        const double *in, double *out, void *stream)
+{
+    if (nout < 0) return HP3D_E_ARG;
+    if (nout == 0) return 0;
+    if (!ptr || !src || !in || !out) return HP3D_E_ARG;
+    long long nblk = (nout + WG - 1) / WG;
+    if (nblk > 0x7fffffffLL) return HP3D_E_ARG;
+    hipLaunchKernelGGL(gather_sum_kernel, dim3((unsigned)nblk), dim3(WG), 0, (hipStream_t)stream,
+                       nout, ptr, src, in, out);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int hp3d_gpu_csr_spmv(long long nrows, const long long *indptr, const int *indices,
+                      const double *vals, const double *x, double *y, void *stream)
+{
+    if (nrows < 0) return HP3D_E_ARG;
+    if (nrows == 0) return 0;
+    if (!indptr || !indices || !vals || !x || !y) return HP3D_E_ARG;
+    long long nblk = (nrows + WG / SPMV_LPR - 1) / (WG / SPMV_LPR);
+    if (nblk > 0x7fffffffLL) return HP3D_E_ARG;
+    hipLaunchKernelGGL(csr_spmv_kernel, dim3((unsigned)nblk), dim3(WG), 0, (hipStream_t)stream,
+                       nrows, indptr, indices, vals, x, y);
     HIPCHK(hipGetLastError());
     return 0;
 }

@@ -11,8 +11,8 @@
 ! passed by address (assumed size), scalars by value, exactly as the C
 ! prototypes expect.
 module hp3d_gpu_mod
-  use iso_c_binding, only: c_int, c_double, c_size_t, c_ptr, c_char, c_null_char, &
-                           c_f_pointer, c_associated
+  use iso_c_binding, only: c_int, c_long_long, c_double, c_size_t, c_ptr, c_char, &
+                           c_null_char, c_f_pointer, c_associated
   implicit none
 
   integer(c_int), parameter :: HP3D_MAXP = 6
@@ -22,7 +22,8 @@ module hp3d_gpu_mod
   integer(c_int), parameter :: HP3D_H1 = 0, HP3D_HCURL = 1
   integer(c_int), parameter :: HP3D_FP64 = 0, HP3D_FP32 = 1
   integer(c_int), parameter :: HP3D_LOAD_ZERO = 0, HP3D_LOAD_SINSINSIN = 1, HP3D_LOAD_CONV = 2
-  integer(c_int), parameter :: HP3D_LOAD_ONE = 3, HP3D_LOAD_CURL_SIN = 10
+  integer(c_int), parameter :: HP3D_LOAD_ONE = 3, HP3D_LOAD_POLY2 = 4, HP3D_LOAD_CURL_SIN = 10
+  integer(c_int), parameter :: HP3D_FIELD_ERROR = 0, HP3D_FIELD_ADJOINT = 1, HP3D_FIELD_L2 = 2
 
   interface
     integer(c_int) function hp3d_gpu_init(device, max_ndof) bind(C, name="hp3d_gpu_init")
@@ -110,10 +111,31 @@ module hp3d_gpu_mod
     end function
 
     integer(c_int) function hp3d_gpu_stc_bwd_batch(space, nel, norder, off_s, aschur, off_v, &
-        bschur, xi, xb, stream) bind(C, name="hp3d_gpu_stc_bwd_batch")
+        bschur, x, stream) bind(C, name="hp3d_gpu_stc_bwd_batch")
       import :: c_int, c_ptr
       integer(c_int), value :: space, nel
-      type(c_ptr), value :: norder, off_s, aschur, off_v, bschur, xi, xb, stream
+      type(c_ptr), value :: norder, off_s, aschur, off_v, bschur, x, stream
+    end function
+
+    integer(c_int) function hp3d_gpu_field_batch(space, mode, nel, norder, nori_e, nori_f, xnod, &
+        ngeo, coef, off_v, u, v, exact, out, r, status, stream) bind(C, name="hp3d_gpu_field_batch")
+      import :: c_int, c_ptr
+      integer(c_int), value :: space, mode, nel, ngeo, exact
+      type(c_ptr), value :: norder, nori_e, nori_f, xnod, coef, off_v, u, v, out, r, status, stream
+    end function
+
+    integer(c_int) function hp3d_gpu_gather_sum(nout, ptr, src, in, out, stream) &
+        bind(C, name="hp3d_gpu_gather_sum")
+      import :: c_int, c_long_long, c_ptr
+      integer(c_long_long), value :: nout
+      type(c_ptr), value :: ptr, src, in, out, stream
+    end function
+
+    integer(c_int) function hp3d_gpu_csr_spmv(nrows, indptr, indices, vals, x, y, stream) &
+        bind(C, name="hp3d_gpu_csr_spmv")
+      import :: c_int, c_long_long, c_ptr
+      integer(c_long_long), value :: nrows
+      type(c_ptr), value :: indptr, indices, vals, x, y, stream
     end function
   end interface
 

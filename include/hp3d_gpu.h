@@ -63,6 +63,7 @@ extern "C" {
 #define HP3D_LOAD_SINSINSIN 1 /* f = 3 pi^2 sin(pi x) sin(pi y) sin(pi z)     */
 #define HP3D_LOAD_CONV 2      /* f = -2 sin(pi x) sinh(pi y)                 */
 #define HP3D_LOAD_ONE 3       /* f = 1                                        */
+#define HP3D_LOAD_POLY2 4     /* f = -Laplace(u), u = x(1-x) y(1-y) z(1-z)    */
 #define HP3D_LOAD_CURL_SIN 10 /* J = (1 + 2 pi^2) sin(pi y) sin(pi z) e_x     */
 
 /* ------------------------------------------------------------ lifecycle */
@@ -172,13 +173,60 @@ int hp3d_gpu_celem_batch(int space, int precision, int nel, const int *norder, c
                          int max_nbubble, void *stream);
 
 /*
- * Back substitution for nel elements (stc_bwd, stc.F90:664-679):
- * xb = bschur - aschur * xi, with xi at off_v[e] in `xi` and xb at off_v[e]
- * in `xb` (device pointers).
+ * Back substitution for nel elements (stc_bwd_wrapper + stc_bwd + stc_solout,
+ * stc.F90:529-679): element e's dof vector sits at x + off_v[e] in element
+ * order; its interface part x[0:ni_e] is read and its bubble part
+ * x[ni_e:N_e] = BSchur - ASchur x[0:ni_e] is written (device pointers;
+ * bschur at off_v[e] as hp3d_gpu_stc_batch writes it).
  */
 int hp3d_gpu_stc_bwd_batch(int space, int nel, const int *norder, const long long *off_s,
                            const double *aschur, const long long *off_v, const double *bschur,
-                           const double *xi, double *xb, void *stream);
+                           double *x, void *stream);
+
+/* ------------------------------------------------ solution-level entries */
+
+#define HP3D_FIELD_ERROR 0   /* out[2e] = error^2, out[2e+1] = exact norm^2       */
+#define HP3D_FIELD_ADJOINT 1 /* out[e] = -coef_e int grad v . grad u  (H(curl): v.u) */
+#define HP3D_FIELD_L2 2      /* out[e] = int u . u;  r[off_v[e]+k] = int phi_k . u  */
+
+/*
+ * Integrals of finite-element fields given by element dof vectors (device
+ * pointers; u, v at off_v[e], element order, full N_e including bubbles):
+ *   HP3D_FIELD_ERROR    element_error (element/util/compute_error.F90:226-400)
+ *                       against the manufactured solution `exact` (a load
+ *                       selector: SINSINSIN, CONV, POLY2 for H1, CURL_SIN for
+ *                       H(curl)): H1 norm (H(curl) norm) squared of u - u_h and
+ *                       of u, quadrature of set_3Dint with INTEGRATION = 2
+ *   HP3D_FIELD_ADJOINT  -lambda_e^T K_e u_e for K_e linear in coef_e, with
+ *                       lambda = v: dJ/dtheta_e when coef_e = exp(theta_e)
+ *                       (no counterpart in the reference)
+ *   HP3D_FIELD_L2       int |u_h|^2 and r_e = M_e u_e (element mass matrix)
+ * status[e] (may be NULL): 0, 1 (negative Jacobian) or HP3D_E_ORDER.
+ */
+int hp3d_gpu_field_batch(int space, int mode, int nel, const int *norder, const int *nori_e,
+                         const int *nori_f, const double *xnod, int ngeo, const double *coef,
+                         const long long *off_v, const double *u, const double *v, int exact,
+                         double *out, double *r, int *status, void *stream);
+
+/*
+ * Deterministic segmented sum out[t] = sum_{ptr[t] <= j < ptr[t+1]} in[src[j]]
+ * (device pointers).  With src the positions of the element contributions to
+ * each global nonzero (or right-hand-side entry) in the packed condensed
+ * outputs, this assembles the element systems into CSR values: the triplet
+ * accumulation of mumps_sc.F90:266-323 (duplicates summed), done on device.
+ */
+int hp3d_gpu_gather_sum(long long nout, const long long *ptr, const long long *src,
+                        const double *in, double *out, void *stream);
+
+/*
+ * y = A x for the assembled global matrix in CSR form (indptr int64,
+ * indices int32, FP64 values; device pointers): the operator of the
+ * iterative solve that stands in for the reference's MUMPS factorisation
+ * (the dmumps analysis, factorisation and solve of mumps_sc.F90:368-426) on meshes too large for a host
+ * direct solve.  No counterpart in the reference.
+ */
+int hp3d_gpu_csr_spmv(long long nrows, const long long *indptr, const int *indices,
+                      const double *vals, const double *x, double *y, void *stream);
 
 #ifdef __cplusplus
 }

@@ -22,6 +22,7 @@ MAXBRICKE = 882
 MAX_NINT3 = 392
 
 LOAD_ZERO, LOAD_SINSINSIN, LOAD_CONV, LOAD_ONE, LOAD_CURL_SIN = 0, 1, 2, 3, 10
+LOAD_POLY2 = 4
 
 _i = ctypes.c_int
 _d = ctypes.c_double
@@ -63,6 +64,11 @@ def _load():
         "orc_stc_fwd_gen": (_i, [_i, _i, _p, _i, _p, _p, _p]),
         "orc_stc_fwd_herm": (_i, [_i, _i, _p, _i, _p, _p, _p]),
         "orc_stc_bwd": (None, [_i, _i, _p, _p, _p, _p]),
+        "orc_set_3dint": (None, [_p, _i, _p, _p, _p]),
+        "orc_exact_h1": (None, [_i, _p, _p, _p]),
+        "orc_exact_hcurl": (None, [_i, _p, _p, _p]),
+        "orc_element_error": (_i, [_i, _p, _p, _p, _p, _i, _p, _i, _p, _p]),
+        "orc_elem_mass": (_i, [_i, _p, _p, _p, _p, _i, _p, _i]),
         "orc_element_loop": (_i, [_i, _i, _p, _p, _p, _p, _i, _p, _i, _i,
                                   _p, _p, _p, _p, _p, _p, _i, _i]),
     }
@@ -329,3 +335,54 @@ def element_loop(space, norder, nori_e, nori_f, xnod, coef, load, ldk, want_k=Tr
                               _ptr(g("bschur")), 1 if herm else 0, nthreads)
     out["status"] = st
     return out
+
+
+def set_3dint(norder, integration=2):
+    """set_3Dint (no orientation swap) with INTEGRATION = integration."""
+    xi = np.zeros((MAX_NINT3 + 200, 3))
+    w = np.zeros(MAX_NINT3 + 200)
+    n = ctypes.c_int()
+    lib.orc_set_3dint(_ptr(_i32(norder, 19)), int(integration), ctypes.byref(n), _ptr(xi), _ptr(w))
+    return xi[:n.value].copy(), w[:n.value].copy()
+
+
+def exact_h1(sel, x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    u = np.zeros(1)
+    du = np.zeros(3)
+    lib.orc_exact_h1(int(sel), _ptr(x), _ptr(u), _ptr(du))
+    return u[0], du
+
+
+def exact_hcurl(sel, x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    e = np.zeros(3)
+    c = np.zeros(3)
+    lib.orc_exact_hcurl(int(sel), _ptr(x), _ptr(e), _ptr(c))
+    return e, c
+
+
+def element_error(space, norder, nori_e, nori_f, xnod, dof, sel):
+    """element_error: (squared error, squared exact norm) of one element.
+    xnod is 3 x ngeo; dof the element's full dof vector (element order)."""
+    xnod = np.asfortranarray(xnod, dtype=np.float64)
+    dof = np.ascontiguousarray(dof, dtype=np.float64)
+    err = np.zeros(1)
+    nrm = np.zeros(1)
+    st = lib.orc_element_error(int(space), _ptr(_i32(norder, 19)), _ptr(_i32(nori_e, 12)),
+                               _ptr(_i32(nori_f, 6)), _ptr(xnod), xnod.shape[1], _ptr(dof),
+                               int(sel), _ptr(err), _ptr(nrm))
+    if st < 0:
+        raise ValueError("element_error failed: %d" % st)
+    return err[0], nrm[0]
+
+
+def elem_mass(space, norder, nori_e, nori_f, xnod):
+    xnod = np.asfortranarray(xnod, dtype=np.float64)
+    n = ndof_h1(norder) if space == 0 else ndof_hcurl(norder)
+    m = np.zeros((n, n), order="F")
+    st = lib.orc_elem_mass(int(space), _ptr(_i32(norder, 19)), _ptr(_i32(nori_e, 12)),
+                           _ptr(_i32(nori_f, 6)), _ptr(xnod), xnod.shape[1], _ptr(m), n)
+    if st < 0:
+        raise ValueError("elem_mass failed: %d" % st)
+    return np.array(m)

@@ -376,9 +376,12 @@ void orc_find_order_loc(const int *norder, const int *nori_f, int *norder_loc)
     }
 }
 
-/* per-direction point counts of set_3Dint_aux, brick branch (set_3D_int.F90:203-244),
- * INTEGRATION = 0 (files/control/control_0) */
-static void brick_nint(const int *nl, int *nint3)
+/* per-direction point counts of set_3Dint_aux, brick branch (set_3D_int.F90:203-244);
+ * INTEGRATION = 0 (files/control/control_0) for the element routines, 2 in
+ * element_error (compute_error.F90:308-310) */
+static void brick_nint_int(const int *nl, int integration, int *nint3);
+static void brick_nint(const int *nl, int *nint3) { brick_nint_int(nl, 0, nint3); }
+static void brick_nint_int(const int *nl, int integration, int *nint3)
 {
     int nx = 0, ny = 0, nz = 0, hv[2], b[3];
     for (int i = 1; i <= 19; i++) {
@@ -410,6 +413,9 @@ static void brick_nint(const int *nl, int *nint3)
             break;
         }
     }
+    nx += integration;
+    ny += integration;
+    nz += integration;
     nx = nx < ORC_MAXP ? nx : ORC_MAXP;
     ny = ny < ORC_MAXP ? ny : ORC_MAXP;
     nz = nz < ORC_MAXP ? nz : ORC_MAXP;
@@ -425,6 +431,27 @@ void orc_set_3d_int(const int *norder, const int *nori_f, int *nint, double *xil
     double gx[3][10], gw[3][10];
     orc_find_order_loc(norder, nori_f, nl);
     brick_nint(nl, n3);
+    for (int d = 0; d < 3; d++) orc_gauss1d(n3[d], gx[d], gw[d]);
+    int l = 0;
+    for (int l3 = 0; l3 < n3[2]; l3++)
+        for (int l2 = 0; l2 < n3[1]; l2++)
+            for (int l1 = 0; l1 < n3[0]; l1++) {
+                xiloc[3 * l + 0] = gx[0][l1];
+                xiloc[3 * l + 1] = gx[1][l2];
+                xiloc[3 * l + 2] = gx[2][l3];
+                waloc[l] = gw[0][l1] * gw[1][l2] * gw[2][l3];
+                l++;
+            }
+    *nint = l;
+}
+
+/* set_3Dint (set_3D_int.F90:8-21): the same brick rule from the face orders
+ * as given (no find_order_loc swap), with INTEGRATION = `integration` */
+void orc_set_3dint(const int *norder, int integration, int *nint, double *xiloc, double *waloc)
+{
+    int n3[3];
+    double gx[3][10], gw[3][10];
+    brick_nint_int(norder, integration, n3);
     for (int d = 0; d < 3; d++) orc_gauss1d(n3[d], gx[d], gw[d]);
     int l = 0;
     for (int l3 = 0; l3 < n3[2]; l3++)
@@ -718,6 +745,11 @@ void orc_getf_h1(int load, const double *x, double *f)
 {
     const double pi = 3.14159265358979323846;
     switch (load) {
+    case ORC_LOAD_POLY2: {
+        double a = x[0] * (1.0 - x[0]), b = x[1] * (1.0 - x[1]), c = x[2] * (1.0 - x[2]);
+        *f = 2.0 * (b * c + a * c + a * b);
+        break;
+    }
     case ORC_LOAD_SINSINSIN:
         *f = 3.0 * pi * pi * sin(pi * x[0]) * sin(pi * x[1]) * sin(pi * x[2]);
         break;
@@ -1092,5 +1124,167 @@ int orc_element_loop(int space, int nel, const int *norder, const int *nori_e, c
         free(aloc);
         free(bloc);
     }
+    return status;
+}
+
+/* ------------------------------------------- solution-level element routines */
+
+/* exact solutions of the manufactured problems, value and gradient
+ * (problems' exact routine: POISSON/GALERKIN/common/exact.F90:30-77,
+ * test/conv_pois.F90 exact; load selectors name the solution they belong to) */
+void orc_exact_h1(int sel, const double *x, double *u, double *du)
+{
+    const double pi = 3.14159265358979323846;
+    double sx = sin(pi * x[0]), sy = sin(pi * x[1]), sz = sin(pi * x[2]);
+    double cx = cos(pi * x[0]), cy = cos(pi * x[1]), cz = cos(pi * x[2]);
+    *u = du[0] = du[1] = du[2] = 0.0;
+    switch (sel) {
+    case ORC_LOAD_SINSINSIN:
+        *u = sx * sy * sz;
+        du[0] = pi * cx * sy * sz;
+        du[1] = pi * sx * cy * sz;
+        du[2] = pi * sx * sy * cz;
+        break;
+    case ORC_LOAD_CONV: {
+        double sh = sinh(pi * x[1]), ch = cosh(pi * x[1]);
+        *u = sx * sh * x[2] * x[2];
+        du[0] = pi * cx * sh * x[2] * x[2];
+        du[1] = pi * sx * ch * x[2] * x[2];
+        du[2] = 2.0 * sx * sh * x[2];
+        break;
+    }
+    case ORC_LOAD_POLY2: {
+        double a = x[0] * (1.0 - x[0]), b = x[1] * (1.0 - x[1]), c = x[2] * (1.0 - x[2]);
+        *u = a * b * c;
+        du[0] = (1.0 - 2.0 * x[0]) * b * c;
+        du[1] = a * (1.0 - 2.0 * x[1]) * c;
+        du[2] = a * b * (1.0 - 2.0 * x[2]);
+        break;
+    }
+    default:
+        break;
+    }
+}
+
+/* E* = (sin(pi y) sin(pi z), 0, 0) and its curl (0, pi sin(pi y) cos(pi z), -pi cos(pi y) sin(pi z)) */
+void orc_exact_hcurl(int sel, const double *x, double *e, double *curl)
+{
+    const double pi = 3.14159265358979323846;
+    for (int i = 0; i < 3; i++) e[i] = curl[i] = 0.0;
+    if (sel == ORC_LOAD_CURL_SIN) {
+        e[0] = sin(pi * x[1]) * sin(pi * x[2]);
+        curl[1] = pi * sin(pi * x[1]) * cos(pi * x[2]);
+        curl[2] = -pi * cos(pi * x[1]) * sin(pi * x[2]);
+    }
+}
+
+/* element_error, H1 and H(curl) branches (element/util/compute_error.F90:226-400):
+ * quadrature of set_3Dint with INTEGRATION = 2, soleval of the element dofs,
+ * errorH = int |grad(u - u_h)|^2 + |u - u_h|^2 (rnormH the same of u), and for
+ * H(curl) errorE = int |curl(E - E_h)|^2 + |E - E_h|^2 (rnormE of E). */
+int orc_element_error(int space, const int *norder, const int *nori_e, const int *nori_f,
+                      const double *xnod, int ngeo, const double *dof, int sel, double *err,
+                      double *rnorm)
+{
+    if (orc_checkorder(norder) < 0) return -1;
+    int nh = orc_ndof_h1(norder);
+    if (ngeo > nh) return -2;
+    int nint, status = 0;
+    double xiloc[3 * ORC_MAX_NINT3], waloc[ORC_MAX_NINT3];
+    orc_set_3dint(norder, 2, &nint, xiloc, waloc);
+    double shap[ORC_MAXBRICKH], grad[3 * ORC_MAXBRICKH];
+    double *she = (double *)malloc(sizeof(double) * 3 * ORC_MAXBRICKE);
+    double *cue = (double *)malloc(sizeof(double) * 3 * ORC_MAXBRICKE);
+    double e2 = 0.0, r2 = 0.0;
+    for (int l = 0; l < nint; l++) {
+        int nrh, nre;
+        double x[3], dxdxi[9], dxidx[9], rjac;
+        orc_shape3dh_hexa(&xiloc[3 * l], norder, nori_e, nori_f, &nrh, shap, grad);
+        if (orc_geom3d(xnod, ngeo, shap, grad, x, dxdxi, dxidx, &rjac)) status = 1;
+        double weight = waloc[l] * rjac;
+        if (space == 0) {
+            double uh = 0.0, gref[3] = {0.0, 0.0, 0.0}, duh[3], u, du[3];
+            for (int k = 0; k < nrh; k++) {
+                uh += dof[k] * shap[k];
+                for (int i = 0; i < 3; i++) gref[i] += dof[k] * grad[3 * k + i];
+            }
+            for (int d = 0; d < 3; d++)
+                duh[d] = gref[0] * dxidx[0 + 3 * d] + gref[1] * dxidx[1 + 3 * d] + gref[2] * dxidx[2 + 3 * d];
+            orc_exact_h1(sel, x, &u, du);
+            for (int j = 0; j < 3; j++) {
+                e2 += (du[j] - duh[j]) * (du[j] - duh[j]) * weight;
+                r2 += du[j] * du[j] * weight;
+            }
+            e2 += (u - uh) * (u - uh) * weight;
+            r2 += u * u * weight;
+        } else {
+            orc_shape3de_hexa(&xiloc[3 * l], norder, nori_e, nori_f, &nre, she, cue);
+            double eref[3] = {0.0, 0.0, 0.0}, cref[3] = {0.0, 0.0, 0.0}, eh[3], ch[3], ex[3], cx[3];
+            for (int k = 0; k < nre; k++)
+                for (int i = 0; i < 3; i++) {
+                    eref[i] += dof[k] * she[3 * k + i];
+                    cref[i] += dof[k] * cue[3 * k + i];
+                }
+            for (int d = 0; d < 3; d++) {
+                eh[d] = eref[0] * dxidx[0 + 3 * d] + eref[1] * dxidx[1 + 3 * d] + eref[2] * dxidx[2 + 3 * d];
+                ch[d] = (dxdxi[d + 0] * cref[0] + dxdxi[d + 3] * cref[1] + dxdxi[d + 6] * cref[2]) / rjac;
+            }
+            orc_exact_hcurl(sel, x, ex, cx);
+            for (int j = 0; j < 3; j++) {
+                e2 += ((cx[j] - ch[j]) * (cx[j] - ch[j]) + (ex[j] - eh[j]) * (ex[j] - eh[j])) * weight;
+                r2 += (cx[j] * cx[j] + ex[j] * ex[j]) * weight;
+            }
+        }
+    }
+    free(she);
+    free(cue);
+    *err = e2;
+    *rnorm = r2;
+    return status;
+}
+
+/* Element mass matrix M_e (H1: int phi_k phi_l; H(curl): int E_k . E_l) with
+ * the element quadrature of elem_opt (set_3D_int, INTEGRATION = 0), column-major
+ * with leading dimension ld.  Used for the L2 objective of the adjoint
+ * problem: J_e = 1/2 d^T M_e d and its gradient M_e d. */
+int orc_elem_mass(int space, const int *norder, const int *nori_e, const int *nori_f,
+                  const double *xnod, int ngeo, double *m, int ld)
+{
+    if (orc_checkorder(norder) < 0) return -1;
+    int nh = orc_ndof_h1(norder), n = space == 0 ? nh : orc_ndof_hcurl(norder);
+    if (ngeo > nh) return -2;
+    int nint, status = 0;
+    double xiloc[3 * ORC_MAX_NINT3], waloc[ORC_MAX_NINT3];
+    orc_set_3d_int(norder, nori_f, &nint, xiloc, waloc);
+    double shap[ORC_MAXBRICKH], grad[3 * ORC_MAXBRICKH];
+    double *she = (double *)malloc(sizeof(double) * 3 * ORC_MAXBRICKE);
+    double *cue = (double *)malloc(sizeof(double) * 3 * ORC_MAXBRICKE);
+    double *f = (double *)malloc(sizeof(double) * 3 * ORC_MAXBRICKE);
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) m[i + (size_t)j * ld] = 0.0;
+    for (int l = 0; l < nint; l++) {
+        int nrh, nre;
+        double x[3], dxdxi[9], dxidx[9], rjac;
+        orc_shape3dh_hexa(&xiloc[3 * l], norder, nori_e, nori_f, &nrh, shap, grad);
+        if (orc_geom3d(xnod, ngeo, shap, grad, x, dxdxi, dxidx, &rjac)) status = 1;
+        double weight = waloc[l] * rjac;
+        if (space == 0) {
+            for (int j = 0; j < n; j++)
+                for (int i = 0; i < n; i++) m[i + (size_t)j * ld] += shap[i] * shap[j] * weight;
+        } else {
+            orc_shape3de_hexa(&xiloc[3 * l], norder, nori_e, nori_f, &nre, she, cue);
+            for (int k = 0; k < n; k++)
+                for (int d = 0; d < 3; d++)
+                    f[3 * k + d] = she[3 * k + 0] * dxidx[0 + 3 * d] + she[3 * k + 1] * dxidx[1 + 3 * d] +
+                                   she[3 * k + 2] * dxidx[2 + 3 * d];
+            for (int j = 0; j < n; j++)
+                for (int i = 0; i < n; i++)
+                    m[i + (size_t)j * ld] += (f[3 * i] * f[3 * j] + f[3 * i + 1] * f[3 * j + 1] +
+                                              f[3 * i + 2] * f[3 * j + 2]) * weight;
+        }
+    }
+    free(she);
+    free(cue);
+    free(f);
     return status;
 }

@@ -32,6 +32,7 @@ extern "C" {
 #define ORC_LOAD_SINSINSIN 1 /* f = 3 pi^2 sin(pi x) sin(pi y) sin(pi z) */
 #define ORC_LOAD_CONV 2      /* f = -2 sin(pi x) sinh(pi y)  (u = sin(pi x) sinh(pi y) z^2) */
 #define ORC_LOAD_ONE 3       /* f = 1 */
+#define ORC_LOAD_POLY2 4     /* f = -Laplace(u), u = x(1-x) y(1-y) z(1-z) */
 #define ORC_LOAD_CURL_SIN 10 /* J = (1+2 pi^2) sin(pi y) sin(pi z) e_x */
 
 void orc_decod(int nick, int mod, int n, int *narr);
@@ -44,6 +45,8 @@ void orc_poly_ilegendre(double x, double t, int nord, int idec, double *l, doubl
 void orc_gauss1d(int n, double *x, double *w);
 /* brick quadrature (set_3D_int.F90) */
 void orc_set_3d_int(const int *norder, const int *nori_f, int *nint, double *xiloc, double *waloc);
+/* set_3Dint (no orientation swap) with a given INTEGRATION */
+void orc_set_3dint(const int *norder, int integration, int *nint, double *xiloc, double *waloc);
 void orc_find_order_loc(const int *norder, const int *nori_f, int *norder_loc);
 
 /* element dof counts (celndof.F90 / element_data.F90 ndof_nod, hexahedron) */
@@ -84,6 +87,17 @@ int orc_stc_fwd_herm(int ni, int nb, double *aloc, int ld, double *bloc, double 
 /* back substitution (stc.F90 stc_bwd): xb = bschur - aschur * xi */
 void orc_stc_bwd(int ni, int nb, const double *aschur, const double *bschur, const double *xi,
                  double *xb);
+
+/* manufactured solutions (selector = the load selector of the same problem) */
+void orc_exact_h1(int sel, const double *x, double *u, double *du);
+void orc_exact_hcurl(int sel, const double *x, double *e, double *curl);
+/* element_error (compute_error.F90:226-400): squared error and exact norm */
+int orc_element_error(int space, const int *norder, const int *nori_e, const int *nori_f,
+                      const double *xnod, int ngeo, const double *dof, int sel, double *err,
+                      double *rnorm);
+/* element mass matrix (H1 or H(curl)), element quadrature */
+int orc_elem_mass(int space, const int *norder, const int *nori_e, const int *nori_f,
+                  const double *xnod, int ngeo, double *m, int ld);
 
 /*
  * Batched element loop, the mumps_sc Step 2 OpenMP loop (omp_mumps/mumps_sc.F90:224-342)

@@ -1,0 +1,132 @@
+"""GPU solution path through the C ABI: field kernel (element_error, L2,
+adjoint integrands), device assembly, solve, back substitution and the
+adjoint gradient, against the FP64 CPU reference built from the oracle."""
+import numpy as np
+import pytest
+
+import oracle
+from hp3d_amd import mesh
+
+import solution_ref as sr
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / (nb if nb > 0 else 1.0)
+
+
+def element_vec(batch, u, e):
+    i = int(batch.iperm[e])
+    o = int(batch.off_v_h[i])
+    return u[o:o + int(batch._n[i])].cpu().numpy()
+
+
+@pytest.mark.parametrize("space", [0, 1])
+def test_field_kernel_modes_against_oracle(gpu, space):
+    import torch
+
+    import hp3d_amd
+    from hp3d_amd import FIELD_ADJOINT, FIELD_ERROR, FIELD_L2
+    from hp3d_amd.solve import field
+
+    rng = np.random.default_rng(3)
+    porder = rng.integers(1, 5, size=(12, 3)) if space == 0 else 2
+    m = mesh.structured((3, 2, 2), porder=porder, perturb=0.12, mesh_seed=4, perm_seed=5,
+                        coef=np.exp(rng.normal(size=12)))
+    batch = hp3d_amd.ElementBatch.from_mesh(space, m)
+    u = torch.as_tensor(rng.normal(size=batch.size_v), device="cuda")
+    v = torch.as_tensor(rng.normal(size=batch.size_v), device="cuda")
+    sel = oracle.LOAD_SINSINSIN if space == 0 else oracle.LOAD_CURL_SIN
+    err, _ = field(batch, FIELD_ERROR, u, exact=sel)
+    l2, r = field(batch, FIELD_L2, u, with_r=True)
+    adj, _ = field(batch, FIELD_ADJOINT, u, v=v)
+    torch.cuda.synchronize()
+    err, l2, adj = err.cpu().numpy(), l2.cpu().numpy(), adj.cpu().numpy()
+    xn = m.xnod()
+    for e in range(m.nel):
+        i = int(batch.iperm[e])
+        ue, ve = element_vec(batch, u, e), element_vec(batch, v, e)
+        de, dn = oracle.element_error(space, m.norder[e], m.nori_e[e], m.nori_f[e], xn[e].T, ue, sel)
+        assert rel(err[i], [de, dn]) < 1e-12
+        mm = oracle.elem_mass(space, m.norder[e], m.nori_e[e], m.nori_f[e], xn[e].T)
+        assert abs(l2[i] - ue @ mm @ ue) < 1e-12 * abs(ue @ mm @ ue)
+        assert rel(element_vec(batch, r, e), mm @ ue) < 1e-12
+        if space == 0:
+            k, _, _ = oracle.elem_h1(m.norder[e], m.nori_e[e], m.nori_f[e], xn[e].T, kappa=m.coef[e])
+        else:
+            k = m.coef[e] * mm       # the eps-scaled mass block is the coef-linear part
+        ref = -ve @ k @ ue
+        assert abs(adj[i] - ref) < 1e-11 * np.sqrt((ve @ k @ ve) * (ue @ k @ ue))
+
+
+@pytest.mark.parametrize("space,p,load", [(0, 2, oracle.LOAD_SINSINSIN), (1, 2, oracle.LOAD_CURL_SIN)])
+def test_solution_matches_cpu_reference(gpu, space, p, load):
+    from hp3d_amd.solve import solve_problem
+
+    coef = np.exp(np.random.default_rng(7).normal(0, 0.5, 27))
+    m = mesh.structured((3, 3, 3), porder=p, perturb=0.1, mesh_seed=6, perm_seed=8, coef=coef)
+    g = solve_problem(m, space, load)
+    c = sr.solve(m, space, load)
+    assert rel(g["x"].cpu().numpy(), c["x"]) < 1e-11
+    for e in range(m.nel):
+        assert rel(element_vec(g["batch"], g["u"], e), c["u"][e]) < 1e-10
+    assert abs(g["error2"] - c["error2"]) < 1e-10 * c["error2"]
+    assert abs(g["norm2"] - c["norm2"]) < 1e-12 * c["norm2"]
+
+
+def test_poly_exact_and_convergence_on_gpu(gpu):
+    from hp3d_amd.solve import solve_problem
+
+    m = mesh.structured((3, 3, 3), porder=2, perm_seed=3)
+    r = solve_problem(m, 0, oracle.LOAD_POLY2)
+    assert np.sqrt(r["error2"]) < 1e-13
+    errs, errs32 = [], []
+    for nx in (2, 4, 8):
+        m = mesh.structured((nx,) * 3, porder=2, perm_seed=nx)
+        errs.append(np.sqrt(solve_problem(m, 0, oracle.LOAD_SINSINSIN)["error2"]))
+        errs32.append(np.sqrt(solve_problem(m, 0, oracle.LOAD_SINSINSIN, precision=1)["error2"]))
+    rate = np.log(errs[-2] / errs[-1]) / np.log(2.0)
+    assert 1.95 <= rate <= 2.05
+    # FP32 contraction leaves the discretisation error unchanged
+    assert np.max(np.abs(np.array(errs32) / np.array(errs) - 1)) < 1e-5, (errs, errs32)
+
+
+def test_pcg_matches_direct_solve(gpu):
+    from hp3d_amd.solve import solve_problem
+
+    coef = np.exp(np.random.default_rng(2).normal(0, 0.5, 125))
+    m = mesh.structured((5, 5, 5), porder=3, perturb=0.1, mesh_seed=3, perm_seed=4, coef=coef)
+    d = solve_problem(m, 0, oracle.LOAD_SINSINSIN, solver="direct")
+    c = solve_problem(m, 0, oracle.LOAD_SINSINSIN, solver="pcg", rtol=1e-13)
+    assert rel(c["x"].cpu().numpy(), d["x"].cpu().numpy()) < 1e-9
+    assert 0 < c["solver"].iterations < 20000
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_adjoint_gradient(gpu, precision):
+    import torch
+
+    from hp3d_amd import adjoint
+
+    m = mesh.structured((3, 3, 2), porder=2, perturb=0.1, mesh_seed=1, perm_seed=2)
+    rng = np.random.default_rng(0)
+    th, tt = rng.normal(0, 0.5, m.nel), rng.normal(0, 0.5, m.nel)
+    prob = adjoint.AdjointProblem(m, tt, precision=precision)
+    theta = torch.tensor(th, dtype=torch.float64, requires_grad=True)
+    J = adjoint.objective(theta, prob)
+    J.backward()
+    g = theta.grad.numpy()
+    Jc, gc = sr.adjoint(m, th, tt, oracle.LOAD_SINSINSIN)
+    tol = 1e-10 if precision == 0 else 1e-5
+    assert abs(float(J) - Jc) < tol * Jc
+    assert np.max(np.abs(g - gc)) < tol * np.abs(gc).max()
+    # central differences through the GPU forward (step 1e-6)
+    h = 1e-6
+    for e in (0, 7, 17):
+        tp, tm = th.copy(), th.copy()
+        tp[e] += h
+        tm[e] -= h
+        fd = (prob.objective(tp)[0] - prob.objective(tm)[0]) / (2 * h)
+        assert abs(fd - g[e]) <= (1e-6 if precision == 0 else 1e-3) * np.abs(g).max(), (e, fd, g[e])
