@@ -37,8 +37,8 @@ SIGNATURES = {
     "hp3d_gpu_elem_batch": (_i, [_i, _i, _i, _p, _p, _p, _p, _i, _p, _i, _p, _i, _p, _p, _p,
                                  _p, _i, _p]),
     "hp3d_gpu_stc_batch": (_i, [_i, _i, _i, _p, _p, _i, _p, _p, _p, _p, _i, _p, _p, _p, _p,
-                                _p, _p, _p, _i, _p]),
-    "hp3d_gpu_stc_workspace": (_sz, [_i, _i]),
+                                _p, _p, _p, _i, _i, _p]),
+    "hp3d_gpu_stc_workspace": (_sz, [_i, _i, _i]),
     "hp3d_gpu_celem_batch": (_i, [_i, _i, _i, _p, _p, _p, _p, _i, _p, _i, _p, _i, _p, _p, _p,
                                   _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _p]),
     "hp3d_gpu_stc_bwd_batch": (_i, [_i, _i, _p, _p, _p, _p, _p, _p, _p]),

@@ -7,8 +7,15 @@
 //                 K += A A^T on MFMA (v_mfma_f64_16x16x4f64 or
 //                 v_mfma_f32_16x16x4f32), upper-triangle tiles only, written
 //                 out in both triangles with coalesced column segments.
-//   stc_kernel    one workgroup per element: Cholesky of the bubble block,
-//                 X = Abb^-1 [Abi | Bb], Aii - Aib X and Bi - Aib Xb, FP64.
+//   elem_single_kernel  one workgroup per element with N <= 128: K kept in
+//                 registers over the quadrature, then (nb <= 32) the bubble
+//                 block condensed in the same launch.
+//   stc_blocked_kernel  one workgroup per element: blocked Cholesky of the
+//                 bubble block with the forward substitution carried along,
+//                 Aii - Y^T Y and Bi - Y^T y_b, X = L^-T Y, on FP64 MFMA.
+//   field_kernel  element integrals of finite-element fields (element_error,
+//                 L2 norm and residual, adjoint coefficient gradient).
+//   gather_sum_kernel, csr_spmv_kernel  device assembly and the global operator.
 //   stc_bwd_kernel  xb = BSchur - ASchur xi.
 //
 // The reference computes the same quantities per element on one CPU thread:
@@ -40,7 +47,6 @@ constexpr int NPT = 8;         // padded max Gauss points per direction
 constexpr int MAXQ = (MAXP + 1) * (MAXP + 1) * (MAXP + 1);
 constexpr int MAXN_H1 = (MAXP + 1) * (MAXP + 1) * (MAXP + 1);
 constexpr int MAXN_E = 3 * MAXP * (MAXP + 1) * (MAXP + 1);
-constexpr int STC_LDS_NB = 88; // bubble blocks up to this size are factored in LDS
 
 __constant__ double c_gx[10][10] = HP3D_GAUSS_X;
 __constant__ double c_gw[10][10] = HP3D_GAUSS_W;
@@ -788,8 +794,32 @@ __global__ __launch_bounds__(WG, 2) void elem_single_kernel(ElemArgs a, CondArgs
 
 // ------------------------------------------------------------ condensation
 
+// ------------------------------------------------------ blocked condensation
+//
+// Static condensation of element systems already in HBM (the algebra of
+// stc_fwd_herm, stc.F90:338-416; Abb is SPD for these forms), one workgroup
+// per element, FP64 arithmetic, FP32 or FP64 K in.  Per element a workspace
+// slab holds L (nbp x nbp, row-major, lower) and Y (nbp x ldy, row-major),
+// nbp = nb rounded up to 16, ldy = ni + 1 rounded up to 16:
+//   A  row r of L and Y is column ni + r of K (K is symmetric: the reads and
+//      the writes are both contiguous); Y's column ni is the bubble load; the
+//      padding is an identity block in L and zeros in Y
+//   B  right-looking blocked Cholesky with panels of 16, carrying Y along:
+//      per panel the 16 x 16 diagonal block is factored in LDS, the panel
+//      below is solved (L21 = A21 L11^-T, one row per thread) and staged in
+//      LDS, the panel's rows of Y are solved (Y_k = L11^-1 Y_k, one column per
+//      thread), then L22 -= L21 L21^T and Y_rest -= L21 Y_k on FP64 MFMA
+//      (16 x 16 tiles, waves round robin) -- at the end Y = L^-1 [Abi | Bb]
+//   C  Aii = Kii - Y^T Y and Bi = Bi - Y^T y_b (upper tiles of Y^T Y on MFMA,
+//      operands straight from the slab, both triangles written coalesced)
+//   D  X = L^-T Y by blocked backward substitution (panel solve per column,
+//      then Y_above -= L_panel^T X_panel on MFMA)
+//   E  ASchur = X[:, 0:ni] (nb x ni, leading dimension nb), BSchur = X[:, ni]
+constexpr int SB = 16;
+__host__ __device__ inline int rnd16(int x) { return (x + 15) & ~15; }
+
 struct StcArgs {
-    int space, nel, ld_k, ld_c, nbmax;
+    int space, nel, ld_k, ld_c, nbp_max, ldy_max;
     const int *norder; // order codes, or NULL when nsz gives (n, nb) per element
     const int *nsz;
     const long long *off_k, *off_v, *off_c, *off_s;
@@ -799,12 +829,20 @@ struct StcArgs {
     int *status;
 };
 
-template <typename T>
-__global__ __launch_bounds__(WG) void stc_kernel(StcArgs a)
+__host__ __device__ inline size_t stc_slab(int nbp_max, int ldy_max)
 {
-    __shared__ double lds_l[STC_LDS_NB * STC_LDS_NB];
+    return (size_t)nbp_max * nbp_max + (size_t)nbp_max * ldy_max;
+}
+
+template <typename T, int PLD>
+__global__ __launch_bounds__(WG) void stc_blocked_kernel(StcArgs a)
+{
+    typedef Mfma<double> MD;
+    __shared__ double l11[SB * 17];
+    __shared__ double P[SB * PLD];          // L21 panel, column k at P + k * PLD
+    __shared__ double stage[4 * 16 * 17];
     __shared__ int s_n, s_nb, s_fail;
-    const int e = blockIdx.x, tid = threadIdx.x;
+    const int e = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (tid == 0) {
         int n = 0, nb = 0;
         bool ok = true;
@@ -847,68 +885,215 @@ __global__ __launch_bounds__(WG) void stc_kernel(StcArgs a)
         if (tid == 0 && a.status) a.status[e] = HP3D_E_NOIFACE;
         return;
     }
-    if (nb > STC_LDS_NB && nb > a.nbmax) { // workspace sized for fewer bubbles
+    const int nbp = rnd16(nb), ldy = rnd16(ni + 1);
+    if (nbp > a.nbp_max || ldy > a.ldy_max || nbp - SB > PLD) { // slab or panel too small
         if (tid == 0 && a.status) a.status[e] = HP3D_E_ARG;
         return;
     }
-    double *L = nb <= STC_LDS_NB ? lds_l : a.ws + (size_t)e * a.nbmax * a.nbmax;
-    const int ldl = nb;
-    // Abb (lower triangle) -> L
-    for (int idx = tid; idx < nb * nb; idx += WG) {
-        int i = idx % nb, j = idx / nb;
-        if (i >= j) L[j * ldl + i] = (double)K[(size_t)(ni + j) * ldk + ni + i];
+    double *L = a.ws + (size_t)e * stc_slab(a.nbp_max, a.ldy_max);
+    double *Y = L + (size_t)nbp * nbp;
+
+    // A: gather
+    for (int r = wave; r < nbp; r += WG / 64) {
+        const T *kc = K + (size_t)(ni + r) * ldk;
+        for (int c = lane; c < nbp; c += 64)
+            L[(size_t)r * nbp + c] = r < nb ? (c < nb ? (double)kc[ni + c] : 0.0) : (c == r ? 1.0 : 0.0);
+        for (int c = lane; c < ldy; c += 64) {
+            double v = 0.0;
+            if (r < nb) v = c < ni ? (double)kc[c] : (c == ni ? b[ni + r] : 0.0);
+            Y[(size_t)r * ldy + c] = v;
+        }
     }
     __syncthreads();
-    // right-looking Cholesky, Abb = L L^T
-    for (int j = 0; j < nb; ++j) {
-        if (tid == 0) {
-            double d = L[j * ldl + j];
-            if (!(d > 0.0)) {
-                s_fail = 1;
-                d = 1.0;
+
+    // B: factor and forward substitution
+    const int np = nbp / SB;
+    const int nyc = ldy / SB;
+    for (int p = 0; p < np; ++p) {
+        const int k0 = SB * p;
+        {
+            int i = tid >> 4, j = tid & 15;
+            l11[i * 17 + j] = L[(size_t)(k0 + i) * nbp + k0 + j];
+        }
+        __syncthreads();
+        for (int j = 0; j < SB; ++j) {
+            if (tid == 0) {
+                double d = l11[j * 17 + j];
+                if (!(d > 0.0)) {
+                    s_fail = 1;
+                    d = 1.0;
+                }
+                l11[j * 17 + j] = sqrt(d);
             }
-            L[j * ldl + j] = sqrt(d);
+            __syncthreads();
+            if (tid > j && tid < SB) l11[tid * 17 + j] /= l11[j * 17 + j];
+            __syncthreads();
+            {
+                int i = tid >> 4, m = tid & 15;
+                if (i > j && m > j && m <= i) l11[i * 17 + m] -= l11[i * 17 + j] * l11[m * 17 + j];
+            }
+            __syncthreads();
+        }
+        {
+            int i = tid >> 4, j = tid & 15;
+            if (j <= i) L[(size_t)(k0 + i) * nbp + k0 + j] = l11[i * 17 + j];
+        }
+        const int nr = nbp - k0 - SB;
+        for (int t = tid; t < nr; t += WG) {
+            // keeps the 136 L11 reads inside the loop (hoisted, they cost
+            // 272 VGPRs and the kernel's occupancy)
+            __asm__ volatile("" ::: "memory");
+            const size_t row = (size_t)(k0 + SB + t) * nbp + k0;
+            double x[SB];
+#pragma unroll
+            for (int j = 0; j < SB; ++j) x[j] = L[row + j];
+#pragma unroll
+            for (int j = 0; j < SB; ++j) {
+                double s = x[j];
+#pragma unroll
+                for (int m = 0; m < j; ++m) s -= x[m] * l11[j * 17 + m];
+                x[j] = s / l11[j * 17 + j];
+            }
+#pragma unroll
+            for (int j = 0; j < SB; ++j) {
+                L[row + j] = x[j];
+                P[j * PLD + t] = x[j];
+            }
+        }
+        for (int c = tid; c < ldy; c += WG) {
+            __asm__ volatile("" ::: "memory");
+            double y[SB];
+#pragma unroll
+            for (int j = 0; j < SB; ++j) y[j] = Y[(size_t)(k0 + j) * ldy + c];
+#pragma unroll
+            for (int j = 0; j < SB; ++j) {
+                double s = y[j];
+#pragma unroll
+                for (int m = 0; m < j; ++m) s -= l11[j * 17 + m] * y[m];
+                y[j] = s / l11[j * 17 + j];
+            }
+#pragma unroll
+            for (int j = 0; j < SB; ++j) Y[(size_t)(k0 + j) * ldy + c] = y[j];
         }
         __syncthreads();
-        double djj = L[j * ldl + j];
-        for (int i = j + 1 + tid; i < nb; i += WG) L[j * ldl + i] /= djj;
-        __syncthreads();
-        int m = nb - j - 1;
-        for (int idx = tid; idx < m * m; idx += WG) {
-            int i = j + 1 + idx % m, k = j + 1 + idx / m;
-            if (i >= k) L[k * ldl + i] -= L[j * ldl + i] * L[j * ldl + k];
+        const int nt = nr / SB;
+        const int nL = nt * (nt + 1) / 2, ntile = nL + nt * nyc;
+        for (int t = wave; t < ntile; t += WG / 64) {
+            int I, J;
+            const bool isL = t < nL;
+            if (isL) {
+                block_ij(t, J, I); // I >= J: lower tiles of the trailing block
+            } else {
+                I = (t - nL) / nyc;
+                J = (t - nL) % nyc;
+            }
+            d4_t acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int ks = 0; ks < SB; ks += 4) {
+                const int kk = ks + (lane >> 4);
+                double av = P[kk * PLD + SB * I + (lane & 15)];
+                double bv = isL ? P[kk * PLD + SB * J + (lane & 15)]
+                                : Y[(size_t)(k0 + kk) * ldy + SB * J + (lane & 15)];
+                acc = MD::mma(av, bv, acc);
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int r = k0 + SB + SB * I + MD::row(lane, j);
+                if (isL) L[(size_t)r * nbp + k0 + SB + SB * J + (lane & 15)] -= acc[j];
+                else Y[(size_t)r * ldy + SB * J + (lane & 15)] -= acc[j];
+            }
         }
         __syncthreads();
     }
-    // X = Abb^-1 [Abi | Bb], one column per thread, into ASchur / BSchur
-    double *X = a.aschur + a.off_s[e];
-    double *Xb = a.bschur + a.off_v[e];
-    for (int c = tid; c <= ni; c += WG) {
-        double *x = c < ni ? X + (size_t)c * nb : Xb;
-        for (int i = 0; i < nb; ++i)
-            x[i] = c < ni ? (double)K[(size_t)c * ldk + ni + i] : b[ni + i];
-        for (int i = 0; i < nb; ++i) { // L y = rhs
-            double s = x[i];
-            for (int k = 0; k < i; ++k) s -= L[k * ldl + i] * x[k];
-            x[i] = s / L[i * ldl + i];
-        }
-        for (int i = nb - 1; i >= 0; --i) { // L^T x = y
-            double s = x[i];
-            for (int k = i + 1; k < nb; ++k) s -= L[i * ldl + k] * x[k];
-            x[i] = s / L[i * ldl + i];
+
+    // C: Aii = Kii - Y^T Y, Bi = Bi - Y^T y_b
+    {
+        const int nst = nyc * (nyc + 1) / 2;
+        for (int t = wave; t < nst; t += WG / 64) {
+            int I, J;
+            block_ij(t, I, J); // I <= J
+            d4_t acc = {0.0, 0.0, 0.0, 0.0};
+            for (int k = 0; k < nbp; k += 4) {
+                const double *yr = Y + (size_t)(k + (lane >> 4)) * ldy + (lane & 15);
+                acc = MD::mma(yr[SB * I], yr[SB * J], acc);
+            }
+            const int cj = SB * J + (lane & 15);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int ri = SB * I + MD::row(lane, j);
+                if (cj == ni && ri < ni) bi[ri] = b[ri] - acc[j];
+                double kv = (ri < ni && cj < ni) ? (double)K[(size_t)ri * ldk + cj] : 0.0;
+                acc[j] = kv - acc[j];
+            }
+            write_sym_tile<d4_t, double, MD>(acc, SB * I, SB * J, ni, aii, ldc, stage + wave * 16 * 17, lane);
         }
     }
     __syncthreads();
-    // Aii - Aib X and Bi - Aib Xb, with Aib(i,b) = K(i, ni+b) = K(ni+b, i)
-    for (int idx = tid; idx < ni * (ni + 1); idx += WG) {
-        int i = idx % ni, j = idx / ni;
-        const double *x = j < ni ? X + (size_t)j * nb : Xb;
-        double s = 0.0;
-        for (int q = 0; q < nb; ++q) s += (double)K[(size_t)i * ldk + ni + q] * x[q];
-        if (j < ni) aii[j * ldc + i] = (double)K[(size_t)j * ldk + i] - s;
-        else bi[i] = b[i] - s;
+
+    // D: X = L^-T Y
+    for (int p = np - 1; p >= 0; --p) {
+        const int k0 = SB * p;
+        {
+            int i = tid >> 4, j = tid & 15;
+            l11[i * 17 + j] = L[(size_t)(k0 + i) * nbp + k0 + j];
+        }
+        __syncthreads();
+        for (int c = tid; c < ldy; c += WG) {
+            __asm__ volatile("" ::: "memory");
+            double y[SB];
+#pragma unroll
+            for (int j = 0; j < SB; ++j) y[j] = Y[(size_t)(k0 + j) * ldy + c];
+#pragma unroll
+            for (int j = SB - 1; j >= 0; --j) {
+                double s = y[j];
+#pragma unroll
+                for (int m = j + 1; m < SB; ++m) s -= l11[m * 17 + j] * y[m];
+                y[j] = s / l11[j * 17 + j];
+            }
+#pragma unroll
+            for (int j = 0; j < SB; ++j) Y[(size_t)(k0 + j) * ldy + c] = y[j];
+        }
+        __syncthreads();
+        const int ntile = p * nyc;
+        for (int t = wave; t < ntile; t += WG / 64) {
+            const int I = t / nyc, J = t % nyc;
+            d4_t acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int ks = 0; ks < SB; ks += 4) {
+                const int kk = k0 + ks + (lane >> 4);
+                acc = MD::mma(L[(size_t)kk * nbp + SB * I + (lane & 15)],
+                              Y[(size_t)kk * ldy + SB * J + (lane & 15)], acc);
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                Y[(size_t)(SB * I + MD::row(lane, j)) * ldy + SB * J + (lane & 15)] -= acc[j];
+        }
+        __syncthreads();
     }
+
+    // E: outputs
+    double *X = a.aschur + a.off_s[e];
+    for (int idx = tid; idx < nb * ni; idx += WG) {
+        const int r = idx % nb, c = idx / nb;
+        X[(size_t)c * nb + r] = Y[(size_t)r * ldy + c];
+    }
+    for (int r = tid; r < nb; r += WG) a.bschur[a.off_v[e] + r] = Y[(size_t)r * ldy + ni];
     if (tid == 0 && a.status) a.status[e] = s_fail ? HP3D_E_FACTOR : 0;
+}
+
+// panel buffer lengths (in doubles) of the two instantiations: nb up to 144
+// (all H1 elements, H(curl) to p = 4) and up to 464 (H(curl) p = 6); each
+// length is 16 mod 32 doubles so that the 16 rows an MFMA operand read
+// touches fall in distinct LDS banks
+constexpr int STC_PLD_S = 144, STC_PLD_L = 464;
+
+template <typename T>
+static void launch_stc(StcArgs a, int max_nbubble, hipStream_t st)
+{
+    if (rnd16(max_nbubble) - SB <= STC_PLD_S)
+        hipLaunchKernelGGL((stc_blocked_kernel<T, STC_PLD_S>), dim3(a.nel), dim3(WG), 0, st, a);
+    else
+        hipLaunchKernelGGL((stc_blocked_kernel<T, STC_PLD_L>), dim3(a.nel), dim3(WG), 0, st, a);
 }
 
 struct BwdArgs {
@@ -1259,6 +1444,7 @@ struct Lib {
     int device = -1;
     hipStream_t stream = nullptr;
     int max_ndof = 0;
+    int nbm = 0; // largest bubble block the per-element workspace holds
     // per-element staging (device)
     int *d_codes = nullptr; // norder[19], nori_e[12], nori_f[6]
     double *d_xnod = nullptr, *d_coef = nullptr, *d_k = nullptr, *d_b = nullptr;
@@ -1377,8 +1563,8 @@ int hp3d_gpu_init(int device, int max_ndof)
     HIPCHK(hipMalloc(&g.d_bi, max_ndof * sizeof(double)));
     HIPCHK(hipMalloc(&g.d_as, nn * sizeof(double)));
     HIPCHK(hipMalloc(&g.d_bs, max_ndof * sizeof(double)));
-    int nbm = max_bubbles(HP3D_HCURL, max_ndof);
-    HIPCHK(hipMalloc(&g.d_ws, (size_t)nbm * nbm * sizeof(double)));
+    g.nbm = max_bubbles(HP3D_HCURL, max_ndof);
+    HIPCHK(hipMalloc(&g.d_ws, stc_slab(rnd16(g.nbm), rnd16(max_ndof + 1)) * sizeof(double)));
     HIPCHK(hipMalloc(&g.d_off, sizeof(long long)));
     HIPCHK(hipMemset(g.d_off, 0, sizeof(long long)));
     HIPCHK(hipMalloc(&g.d_status, 2 * sizeof(int)));
@@ -1441,10 +1627,10 @@ int hp3d_gpu_elem_batch(int space, int precision, int nel, const int *norder, co
     return launch_elem(space, precision, a, max_ndof, (hipStream_t)stream);
 }
 
-size_t hp3d_gpu_stc_workspace(int nel, int max_nbubble)
+size_t hp3d_gpu_stc_workspace(int nel, int max_ndof, int max_nbubble)
 {
-    if (nel <= 0 || max_nbubble <= STC_LDS_NB) return 0;
-    return (size_t)nel * max_nbubble * max_nbubble * sizeof(double);
+    if (nel <= 0 || max_nbubble <= 0 || max_ndof <= 0) return 0;
+    return (size_t)nel * stc_slab(rnd16(max_nbubble), rnd16(max_ndof + 1)) * sizeof(double);
 }
 
 int hp3d_gpu_stc_batch(int space, int precision, int nel, const int *norder,
@@ -1452,23 +1638,26 @@ int hp3d_gpu_stc_batch(int space, int precision, int nel, const int *norder,
                        const long long *off_v, const double *bin, const long long *off_c,
                        int ld_c, double *aii, double *bi, const long long *off_s,
                        double *aschur, double *bschur, void *workspace, int *status,
-                       int max_nbubble, void *stream)
+                       int max_ndof, int max_nbubble, void *stream)
 {
     if ((space != HP3D_H1 && space != HP3D_HCURL) || (precision != HP3D_FP64 && precision != HP3D_FP32))
         return HP3D_E_ARG;
-    if (nel < 0 || max_nbubble < 0 || ld_k < 0 || ld_c < 0) return HP3D_E_ARG;
+    if (nel < 0 || max_ndof < 1 || max_ndof > MAXN_E || max_nbubble < 0 || max_nbubble > max_ndof ||
+        ld_k < 0 || ld_c < 0)
+        return HP3D_E_ARG;
     if (nel == 0) return 0;
     if (!norder || !off_k || !kin || !off_v || !bin || !off_c || !aii || !bi || !off_s ||
         !aschur || !bschur)
         return HP3D_E_ARG;
-    if (hp3d_gpu_stc_workspace(nel, max_nbubble) > 0 && !workspace) return HP3D_E_ARG;
+    if (hp3d_gpu_stc_workspace(nel, max_ndof, max_nbubble) > 0 && !workspace) return HP3D_E_ARG;
     if (precision == HP3D_FP32 && (const void *)aii == kin) return HP3D_E_ARG;
     StcArgs a;
     a.space = space;
     a.nel = nel;
     a.ld_k = ld_k;
     a.ld_c = ld_c;
-    a.nbmax = max_nbubble;
+    a.nbp_max = rnd16(max_nbubble);
+    a.ldy_max = rnd16(max_ndof + 1);
     a.norder = norder;
     a.nsz = nullptr;
     a.off_k = off_k;
@@ -1484,8 +1673,8 @@ int hp3d_gpu_stc_batch(int space, int precision, int nel, const int *norder,
     a.ws = (double *)workspace;
     a.status = status;
     hipStream_t st = (hipStream_t)stream;
-    if (precision == HP3D_FP64) hipLaunchKernelGGL(stc_kernel<double>, dim3(nel), dim3(WG), 0, st, a);
-    else hipLaunchKernelGGL(stc_kernel<float>, dim3(nel), dim3(WG), 0, st, a);
+    if (precision == HP3D_FP64) launch_stc<double>(a, max_nbubble, st);
+    else launch_stc<float>(a, max_nbubble, st);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -1535,8 +1724,8 @@ int hp3d_gpu_celem_batch(int space, int precision, int nel, const int *norder, c
     int r = launch_elem(space, precision, a, max_ndof, st);
     if (r) return r;
     return hp3d_gpu_stc_batch(space, precision, nel, norder, off_k, ld_k, kout, off_v, bout, off_c,
-                              0, aii, bi, off_s, aschur, bschur, workspace, status, max_nbubble,
-                              stream);
+                              0, aii, bi, off_s, aschur, bschur, workspace, status, max_ndof,
+                              max_nbubble, stream);
 }
 
 int hp3d_gpu_stc_bwd_batch(int space, int nel, const int *norder, const long long *off_s,
@@ -1667,7 +1856,8 @@ static int stc_run(int space, int n, int nb, const int *d_norder, const int *d_n
     a.nel = 1;
     a.ld_k = 0;
     a.ld_c = 0;
-    a.nbmax = nb;
+    a.nbp_max = rnd16(g.nbm);
+    a.ldy_max = rnd16(g.max_ndof + 1);
     a.norder = d_norder;
     a.nsz = d_nsz;
     a.off_k = a.off_v = a.off_c = a.off_s = g.d_off;
@@ -1680,7 +1870,7 @@ static int stc_run(int space, int n, int nb, const int *d_norder, const int *d_n
     a.ws = g.d_ws;
     a.status = d_status;
     (void)n;
-    hipLaunchKernelGGL(stc_kernel<double>, dim3(1), dim3(WG), 0, g.stream, a);
+    launch_stc<double>(a, nb, g.stream);
     HIPCHK(hipGetLastError());
     return 0;
 }

@@ -226,7 +226,7 @@ class ElementBatch:
         self.off_s = t(self.off_s_h, torch.int64)
         self.status = torch.zeros(self.nel, dtype=torch.int32, device=dev)
         lib = _lib.load()
-        ws = max([lib.hp3d_gpu_stc_workspace(cnt, mnb) for _, cnt, _, mnb in self.bins] + [8])
+        ws = max([lib.hp3d_gpu_stc_workspace(cnt, mnd, mnb) for _, cnt, mnd, mnb in self.bins] + [8])
         self.workspace = torch.empty(max(int(ws) // 8, 1), dtype=torch.float64, device=dev)
 
     def _ptrs(self, start):
@@ -281,13 +281,13 @@ class ElementBatch:
 
         lib = _lib.load()
         st = torch.cuda.current_stream().cuda_stream if stream is None else stream
-        for lo, cnt, _mnd, mnb in self.bins:
+        for lo, cnt, mnd, mnb in self.bins:
             at = self._ptrs(lo)
             rc = lib.hp3d_gpu_stc_batch(
                 self.space, precision, cnt, at(self.norder, 19), at(self.off_k, 1), 0,
                 _tptr(out["k"]), at(self.off_v, 1), _tptr(out["b"]), at(self.off_c, 1), 0,
                 _tptr(out["aii"]), _tptr(out["bi"]), at(self.off_s, 1), _tptr(out["aschur"]),
-                _tptr(out["bschur"]), _tptr(self.workspace), at(self.status, 1), mnb,
+                _tptr(out["bschur"]), _tptr(self.workspace), at(self.status, 1), mnd, mnb,
                 ctypes.c_void_p(st))
             check(rc, "hp3d_gpu_stc_batch")
 

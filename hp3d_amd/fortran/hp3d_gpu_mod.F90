@@ -87,17 +87,17 @@ module hp3d_gpu_mod
 
     integer(c_int) function hp3d_gpu_stc_batch(space, precision, nel, norder, off_k, ld_k, kin, &
         off_v, bin, off_c, ld_c, aii, bi, off_s, aschur, bschur, workspace, status, &
-        max_nbubble, stream) bind(C, name="hp3d_gpu_stc_batch")
+        max_ndof, max_nbubble, stream) bind(C, name="hp3d_gpu_stc_batch")
       import :: c_int, c_ptr
-      integer(c_int), value :: space, precision, nel, ld_k, ld_c, max_nbubble
+      integer(c_int), value :: space, precision, nel, ld_k, ld_c, max_ndof, max_nbubble
       type(c_ptr), value :: norder, off_k, kin, off_v, bin, off_c, aii, bi, off_s
       type(c_ptr), value :: aschur, bschur, workspace, status, stream
     end function
 
-    integer(c_size_t) function hp3d_gpu_stc_workspace(nel, max_nbubble) &
+    integer(c_size_t) function hp3d_gpu_stc_workspace(nel, max_ndof, max_nbubble) &
         bind(C, name="hp3d_gpu_stc_workspace")
       import :: c_int, c_size_t
-      integer(c_int), value :: nel, max_nbubble
+      integer(c_int), value :: nel, max_ndof, max_nbubble
     end function
 
     integer(c_int) function hp3d_gpu_celem_batch(space, precision, nel, norder, nori_e, nori_f, &

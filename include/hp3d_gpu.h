@@ -69,9 +69,9 @@ extern "C" {
 /* ------------------------------------------------------------ lifecycle */
 
 /* Select the device and create the library's stream and pinned staging for
- * the per-element entries.  No allocation happens in later per-element calls
- * unless an element exceeds the staging sized here (max_ndof dofs, 0 = MAXP
- * maximum).  Replaces the THREADPRIVATE workspace setup of mumps_sc Step 2
+ * the per-element entries, sized for elements of up to max_ndof dofs (0 = the
+ * MAXP maximum, 882); later per-element calls allocate nothing and reject a
+ * larger element with HP3D_E_ARG.  Replaces the THREADPRIVATE workspace setup of mumps_sc Step 2
  * (omp_mumps/mumps_sc.F90:228-251). */
 int hp3d_gpu_init(int device, int max_ndof);
 int hp3d_gpu_finalize(void);
@@ -139,19 +139,21 @@ int hp3d_gpu_elem_batch(int space, int precision, int nel, const int *norder, co
  *        precision is FP64, off_c == off_k and ld_c == ld_k: in place as ALOC)
  *   bi   at off_v[e]: condensed load (may alias bin)
  *   aschur at off_s[e] (nb_e x ni_e, leading dimension nb_e), bschur at off_v[e]
- *   workspace: device memory of hp3d_gpu_stc_workspace(nel, max_nbubble) bytes
- *   max_nbubble bounds nb_e over the batch (an element above it reports
- *   HP3D_E_ARG in status[e])
+ *   workspace: device memory of hp3d_gpu_stc_workspace(nel, max_ndof,
+ *   max_nbubble) bytes (one slab per element: the bubble factor and the
+ *   forward-substituted interface columns)
+ *   max_ndof, max_nbubble bound N_e and nb_e over the batch (an element above
+ *   them reports HP3D_E_ARG in status[e])
  */
 int hp3d_gpu_stc_batch(int space, int precision, int nel, const int *norder,
                        const long long *off_k, int ld_k, const void *kin,
                        const long long *off_v, const double *bin, const long long *off_c,
                        int ld_c, double *aii, double *bi, const long long *off_s,
                        double *aschur, double *bschur, void *workspace, int *status,
-                       int max_nbubble, void *stream);
+                       int max_ndof, int max_nbubble, void *stream);
 /* bytes of device workspace the condensation of nel elements with at most
- * max_nbubble bubble dofs needs (0 when their bubble blocks fit in LDS) */
-size_t hp3d_gpu_stc_workspace(int nel, int max_nbubble);
+ * max_ndof dofs and max_nbubble bubble dofs needs (0 when max_nbubble = 0) */
+size_t hp3d_gpu_stc_workspace(int nel, int max_ndof, int max_nbubble);
 
 /*
  * Element matrices, loads and static condensation in one call (the work of
@@ -161,7 +163,8 @@ size_t hp3d_gpu_stc_workspace(int nel, int max_nbubble);
  * element kernel on the register-resident K (no second pass over HBM); kout
  * and bout may then be NULL (K and the full load are not written).  Otherwise
  * K and the load are written and condensed by a second kernel; kout, bout and
- * the hp3d_gpu_stc_workspace(...) workspace are then required.  The condensed
+ * the hp3d_gpu_stc_workspace(nel, max_ndof, max_nbubble) workspace are then
+ * required.  The condensed
  * matrix is packed (leading dimension ni_e).
  */
 int hp3d_gpu_celem_batch(int space, int precision, int nel, const int *norder, const int *nori_e,

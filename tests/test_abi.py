@@ -74,11 +74,13 @@ def test_entries_before_init_and_bad_arguments():
     assert lib.hp3d_gpu_elem_batch(0, 0, 1, None, None, None, None, 8, None, 0, None, 0, None,
                                    None, None, None, 1, None) == _lib.E_ARG
     assert lib.hp3d_gpu_stc_batch(0, 2, 1, None, None, 0, None, None, None, None, 0, None, None,
-                                  None, None, None, None, None, 1, None) == _lib.E_ARG
+                                  None, None, None, None, None, 8, 1, None) == _lib.E_ARG
     assert lib.hp3d_gpu_elem_batch(0, 0, 0, None, None, None, None, 8, None, 0, None, 0, None,
                                    None, None, None, 1, None) == 0  # empty batch
-    assert lib.hp3d_gpu_stc_workspace(10, 27) == 0      # p=4 bubbles factor in LDS
-    assert lib.hp3d_gpu_stc_workspace(10, 240) == 10 * 240 * 240 * 8
+    assert lib.hp3d_gpu_stc_workspace(10, 125, 0) == 0
+    # slab per element: nb x nb factor + nb x (ni + 1) columns, both rounded up to 16
+    assert lib.hp3d_gpu_stc_workspace(10, 125, 27) == 10 * (32 * 32 + 32 * 128) * 8
+    assert lib.hp3d_gpu_stc_workspace(10, 540, 240) == 10 * (240 * 240 + 240 * 544) * 8
 
 
 def test_python_mirror_counts():

@@ -120,13 +120,16 @@ def test_adjoint_gradient(gpu, precision):
     g = theta.grad.numpy()
     Jc, gc = sr.adjoint(m, th, tt, oracle.LOAD_SINSINSIN)
     tol = 1e-10 if precision == 0 else 1e-5
-    assert abs(float(J) - Jc) < tol * Jc
+    assert abs(float(J.detach()) - Jc) < tol * Jc
     assert np.max(np.abs(g - gc)) < tol * np.abs(gc).max()
-    # central differences through the GPU forward (step 1e-6)
+    # central differences (step 1e-6) through the GPU FP64 forward: the
+    # derivative both forms approximate (a 1e-6 step on the FP32 forward would
+    # be buried in the FP32 rounding of K, eps ~ 6e-8)
+    p64 = prob if precision == 0 else adjoint.AdjointProblem(m, tt, precision=0)
     h = 1e-6
     for e in (0, 7, 17):
         tp, tm = th.copy(), th.copy()
         tp[e] += h
         tm[e] -= h
-        fd = (prob.objective(tp)[0] - prob.objective(tm)[0]) / (2 * h)
-        assert abs(fd - g[e]) <= (1e-6 if precision == 0 else 1e-3) * np.abs(g).max(), (e, fd, g[e])
+        fd = (p64.objective(tp)[0] - p64.objective(tm)[0]) / (2 * h)
+        assert abs(fd - g[e]) <= (1e-6 if precision == 0 else 1e-5) * np.abs(g).max(), (e, fd, g[e])
