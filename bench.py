@@ -137,6 +137,80 @@ def cpu_baseline(m, batch, out32, out64, seconds=15.0):
     return res, errors
 
 
+def adjoint_leg(m, nfd):
+    """dJ/dtheta_e for all elements of the 24^3 p = 3 mesh (J = 1/2 ||u - u_t||^2,
+    u_t the solution for a second seeded theta): forward (element kernels with
+    condensation, device assembly, PCG, back substitution, L2 field kernel) and
+    gradient (adjoint condensation and solve, field kernel) timed per form; the
+    gradient kernel checked against -lambda_e^T K_e u_e with the oracle's K_e on
+    a sample of elements, and against central differences (step 1e-6) of the
+    FP64 forward on nfd seeded elements."""
+    import torch
+
+    import hp3d_amd
+    import oracle
+    from hp3d_amd import adjoint
+    from hp3d_amd.solve import field
+
+    theta = np.log(m.coef)
+    theta_t = np.random.default_rng(3004).normal(0.0, 0.5, m.nel)
+    res = {}
+    g64 = None
+    for prec, name in ((hp3d_amd.FP64, "fp64"), (hp3d_amd.FP32, "fp32")):
+        prob = adjoint.AdjointProblem(m, theta_t, precision=prec, solver="pcg", rtol=1e-13)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        J, st = prob.objective(theta)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        its_fwd = st["solver"].iterations
+        g, lam = prob.gradient(st)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        # the gradient kernel alone (HIP events on its stream)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(5):
+            field(prob.batch, hp3d_amd.FIELD_ADJOINT, st["u"], v=lam)
+        e1.record()
+        torch.cuda.synchronize()
+        kms = e0.elapsed_time(e1) / 5
+        b = prob.batch
+        u, lv = st["u"].cpu().numpy(), lam.cpu().numpy()
+        xn = m.xnod()
+        sample = np.random.default_rng(4004).choice(m.nel, 256, replace=False)
+        err = 0.0
+        for e in sample:
+            i = int(b.iperm[e])
+            o, n = int(b.off_v_h[i]), int(b._n[i])
+            k, _, _ = oracle.elem_h1(m.norder[e], m.nori_e[e], m.nori_f[e], xn[e].T,
+                                     kappa=float(np.exp(theta[e])))
+            err = max(err, abs(g[e] + lv[o:o + n] @ k @ u[o:o + n]))
+        entry = {"J": J, "forward_s": t1 - t0, "gradient_s": t2 - t1,
+                 "pcg_iterations": [its_fwd, st["solver"].iterations],
+                 "gradient_kernel_ms": kms, "gradient_kernel_elements_per_s": m.nel / (kms * 1e-3),
+                 "grad_vs_oracle_K_max_rel": err / np.abs(g).max(),
+                 "grad_sample": "256 seeded elements"}
+        if prec == hp3d_amd.FP64:
+            g64 = g
+            fd_err = []
+            x0 = st["x"]
+            for e in np.random.default_rng(4004).choice(m.nel, nfd, replace=False):
+                tp, tm = theta.copy(), theta.copy()
+                tp[e] += 1e-6
+                tm[e] -= 1e-6
+                fd = (prob.objective(tp, x0=x0)[0] - prob.objective(tm, x0=x0)[0]) / 2e-6
+                fd_err.append(abs(fd - g[e]) / np.abs(g).max())
+            entry["fd_elements"] = nfd
+            entry["fd_max_rel"] = max(fd_err) if fd_err else None
+        else:
+            entry["grad_vs_fp64_max_rel"] = float(np.abs(g - g64).max() / np.abs(g64).max())
+        res[name] = entry
+        del prob
+        torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -149,6 +223,8 @@ def main():
                     help="time only the FP32 element+condensation launches (for counter runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run one device-to-device copy of MIB MiB and exit (counter scale)")
+    ap.add_argument("--fd", type=int, default=8,
+                    help="cfg4: elements checked by central differences (step 1e-6)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: element work has no cross-GPU exchange here and "
@@ -254,6 +330,8 @@ def main():
     line["roofline_fp64"] = {"achieved": (fk + fc) / (e64 * 1e-3) / 1e12,
                              "peak": PEAK["fp64"] / 1e12, "unit": "TFLOP/s",
                              "frac": (fk + fc) / (e64 * 1e-3) / PEAK["fp64"]}
+    if args.workload == "cfg4":
+        line["adjoint"] = adjoint_leg(m, args.fd)
     if not args.no_cpu:
         cb, errs = cpu_baseline(m, batch, out32, out64)
         line["cpu_baseline"] = cb
