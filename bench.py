@@ -274,6 +274,15 @@ def main():
     torch.cuda.synchronize()
     sg = (time.perf_counter() - t0) / args.steps
 
+    # BF16 trial: bfloat16 operands, FP32 accumulation, block-tiled element
+    # kernel then the blocked condensation
+    out16 = batch.alloc_outputs(hp3d_amd.BF16)
+    s16, e16 = time_steps(batch, out16, hp3d_amd.BF16, max(2, args.steps // 2), 1)
+    batch.check_status()
+    k16max, k16med = rel_err_per_element(batch, out16["k"], out64["k"], batch._n * batch._n)
+    a16max, _a16med = rel_err_per_element(batch, out16["aii"], out64["aii"], batch._ni * batch._ni)
+    del out16
+
     fk, fc = algorithmic_flops(batch)
     nel = batch.nel
     nb_total = float(batch.nb.sum())
@@ -311,6 +320,9 @@ def main():
                                    "note": "hp3d_gpu_celem per element, host arrays, %d elements"
                                            % min(args.dropin_sample, nel)} if dropin else None),
             "fp32_over_dropin": (value / dropin) if dropin else None,
+            "bf16_trial": {"element_matrices_per_s": nel / s16, "ms_per_step": s16 * 1e3,
+                           "k_err_max": k16max, "k_err_median": k16med, "aii_err_max": a16max,
+                           "note": "bfloat16 operands, FP32 accumulation (two-pass path)"},
         },
         "fp32_vs_fp64_error": {"k_max": kmax, "k_median": kmed, "aii_max": amax,
                                "aii_median": amed},

@@ -10,7 +10,7 @@ objective (adjoint.py) and synthetic meshes carrying hp3D's order and
 orientation codes (mesh.py).
 """
 from . import mesh  # noqa: F401
-from ._lib import (FIELD_ADJOINT, FIELD_ERROR, FIELD_L2, FP32, FP64, H1, HCURL,  # noqa: F401
+from ._lib import (BF16, FIELD_ADJOINT, FIELD_ERROR, FIELD_L2, FP32, FP64, H1, HCURL,  # noqa: F401
                    LOAD_CONV, LOAD_CURL_SIN, LOAD_ONE, LOAD_POLY2, LOAD_SINSINSIN, LOAD_ZERO,
                    Hp3dError, init)
 from .elem import ElementBatch, celem, elem, ndof, stc_fwd  # noqa: F401

@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libhp3d_gpu.so")
 
 H1, HCURL = 0, 1
-FP64, FP32 = 0, 1
+FP64, FP32, BF16 = 0, 1, 2
 LOAD_ZERO, LOAD_SINSINSIN, LOAD_CONV, LOAD_ONE, LOAD_CURL_SIN = 0, 1, 2, 3, 10
 LOAD_POLY2 = 4
 FIELD_ERROR, FIELD_ADJOINT, FIELD_L2 = 0, 1, 2

@@ -384,8 +384,27 @@ __device__ inline void write_sym_tile(V acc, int R, int C, int lim, S *out, size
 
 
 
+// bfloat16 operands for the BF16 trial: round to nearest even, 4 values
+// packed per lane as v_mfma_f32_16x16x16_bf16 takes them (lane l supplies
+// A[l % 16][4 (l / 16) + i] and B[4 (l / 16) + i][l % 16], i = 0..3; the
+// result has the layout of the f32 16x16x4 instruction)
+typedef short s4_t __attribute__((ext_vector_type(4)));
+__device__ inline short f2bf(float f)
+{
+    unsigned u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (short)(u >> 16);
+}
+__device__ inline s4_t bf4(const float *col, int ld, int off)
+{
+    return s4_t{f2bf(col[off]), f2bf(col[ld + off]), f2bf(col[2 * ld + off]), f2bf(col[3 * ld + off])};
+}
+
 // General element kernel: one workgroup per (element, 128 x 128 block of K).
-template <typename T, int SPACE>
+// BF (with T = float): the contraction's operands are rounded to bfloat16 and
+// multiplied on v_mfma_f32_16x16x16_bf16 with FP32 accumulation (the BF16
+// trial); the A chunk is padded from KC to 32 columns with zeros.
+template <typename T, int SPACE, bool BF = false>
 __global__ __launch_bounds__(WG) void elem_kernel(ElemArgs a)
 {
     typedef Mfma<T> M;
@@ -394,13 +413,15 @@ __global__ __launch_bounds__(WG) void elem_kernel(ElemArgs a)
     constexpr int QC = KC / NC;              // points per A chunk
     constexpr int GQ = 64;                   // points per geometry pass
     constexpr int GS = SPACE == 0 ? 10 : 21; // geometry doubles per point
+    constexpr int KCA = BF ? 32 : KC;        // allocated A columns
+    static_assert(!BF || sizeof(T) == 4, "bfloat16 operands come from an FP32 chunk");
 
     __shared__ ElemShared es;
     __shared__ double tv[3][NTAB * NPT], td[3][NTAB * NPT];
     __shared__ unsigned dsp[2 * BR];   // descriptors of the block's rows
     __shared__ unsigned dgeo[MAXN_H1]; // H1 descriptors of the geometry dofs
     __shared__ double geo[GQ * GS];
-    __shared__ T ac[KC * LDA];
+    __shared__ T ac[KCA * LDA];
 
     const int e = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -421,6 +442,8 @@ __global__ __launch_bounds__(WG) void elem_kernel(ElemArgs a)
         dsp[r] = g < n ? (SPACE == 0 ? desc_h1(g, es.o) : desc_hcurl(g, es.o)) : 0u;
     }
     for (int k = tid; k < a.ngeo; k += WG) dgeo[k] = desc_h1(k, es.o);
+    if (BF)
+        for (int i = tid; i < (KCA - KC) * LDA; i += WG) ac[KC * LDA + i] = (T)0;
     __syncthreads();
 
     const double *xn = a.xnod + 3 * (size_t)a.ngeo * e;
@@ -466,16 +489,33 @@ __global__ __launch_bounds__(WG) void elem_kernel(ElemArgs a)
                 for (int c = 0; c < NC; ++c) ac[(NC * q + c) * LDA + r] = (T)out[c];
             }
             __syncthreads();
+            if constexpr (BF) {
 #pragma unroll
-            for (int k0 = 0; k0 < KC; k0 += 4) {
-                const T *col = ac + (k0 + (lane >> 4)) * LDA;
-                T aA = col[tiA * 16 + (lane & 15)], aB = col[tiB * 16 + (lane & 15)];
-                T b[8];
+                for (int k0 = 0; k0 < KCA; k0 += 16) {
+                    const float *col = (const float *)ac + (k0 + 4 * (lane >> 4)) * LDA;
+                    s4_t aA = bf4(col, LDA, tiA * 16 + (lane & 15));
+                    s4_t aB = bf4(col, LDA, tiB * 16 + (lane & 15));
+                    s4_t b[8];
 #pragma unroll
-                for (int tj = 0; tj < 8; ++tj) b[tj] = col[offJ + tj * 16 + (lane & 15)];
+                    for (int tj = 0; tj < 8; ++tj) b[tj] = bf4(col, LDA, offJ + tj * 16 + (lane & 15));
 #pragma unroll
-                for (int t = 0; t < 16; ++t)
-                    if (valid & (1u << t)) acc[t] = M::mma(t < 8 ? aA : aB, b[t & 7], acc[t]);
+                    for (int t = 0; t < 16; ++t)
+                        if (valid & (1u << t))
+                            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(t < 8 ? aA : aB, b[t & 7],
+                                                                              acc[t], 0, 0, 0);
+                }
+            } else {
+#pragma unroll
+                for (int k0 = 0; k0 < KC; k0 += 4) {
+                    const T *col = ac + (k0 + (lane >> 4)) * LDA;
+                    T aA = col[tiA * 16 + (lane & 15)], aB = col[tiB * 16 + (lane & 15)];
+                    T b[8];
+#pragma unroll
+                    for (int tj = 0; tj < 8; ++tj) b[tj] = col[offJ + tj * 16 + (lane & 15)];
+#pragma unroll
+                    for (int t = 0; t < 16; ++t)
+                        if (valid & (1u << t)) acc[t] = M::mma(t < 8 ? aA : aB, b[t & 7], acc[t]);
+                }
             }
             __syncthreads();
         }
@@ -1493,16 +1533,18 @@ static int launch_single(int space, int precision, const ElemArgs &args, const C
 static int launch_elem(int space, int precision, const ElemArgs &args, int max_ndof,
                        hipStream_t st)
 {
-    if (max_ndof <= BR) {
+    if (max_ndof <= BR && precision != HP3D_BF16) {
         CondArgs cd{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
         return launch_single(space, precision, args, cd, st);
     }
     dim3 grid(args.nel, nblocks_for(max_ndof)), block(WG);
     if (space == HP3D_H1) {
         if (precision == HP3D_FP64) hipLaunchKernelGGL((elem_kernel<double, 0>), grid, block, 0, st, args);
+        else if (precision == HP3D_BF16) hipLaunchKernelGGL((elem_kernel<float, 0, true>), grid, block, 0, st, args);
         else hipLaunchKernelGGL((elem_kernel<float, 0>), grid, block, 0, st, args);
     } else {
         if (precision == HP3D_FP64) hipLaunchKernelGGL((elem_kernel<double, 1>), grid, block, 0, st, args);
+        else if (precision == HP3D_BF16) hipLaunchKernelGGL((elem_kernel<float, 1, true>), grid, block, 0, st, args);
         else hipLaunchKernelGGL((elem_kernel<float, 1>), grid, block, 0, st, args);
     }
     HIPCHK(hipGetLastError());
@@ -1600,7 +1642,7 @@ int hp3d_gpu_elem_batch(int space, int precision, int nel, const int *norder, co
                         const long long *off_v, double *bout, int *status, int max_ndof,
                         void *stream)
 {
-    if ((space != HP3D_H1 && space != HP3D_HCURL) || (precision != HP3D_FP64 && precision != HP3D_FP32))
+    if ((space != HP3D_H1 && space != HP3D_HCURL) || (precision != HP3D_FP64 && precision != HP3D_FP32 && precision != HP3D_BF16))
         return HP3D_E_ARG;
     if (nel < 0 || ngeo < 1 || ngeo > MAXN_H1 || max_ndof < 1 || ld_k < 0) return HP3D_E_ARG;
     if (nel == 0) return 0;
@@ -1640,7 +1682,7 @@ int hp3d_gpu_stc_batch(int space, int precision, int nel, const int *norder,
                        double *aschur, double *bschur, void *workspace, int *status,
                        int max_ndof, int max_nbubble, void *stream)
 {
-    if ((space != HP3D_H1 && space != HP3D_HCURL) || (precision != HP3D_FP64 && precision != HP3D_FP32))
+    if ((space != HP3D_H1 && space != HP3D_HCURL) || (precision != HP3D_FP64 && precision != HP3D_FP32 && precision != HP3D_BF16))
         return HP3D_E_ARG;
     if (nel < 0 || max_ndof < 1 || max_ndof > MAXN_E || max_nbubble < 0 || max_nbubble > max_ndof ||
         ld_k < 0 || ld_c < 0)
@@ -1687,7 +1729,7 @@ int hp3d_gpu_celem_batch(int space, int precision, int nel, const int *norder, c
                          double *bschur, void *workspace, int *status, int max_ndof,
                          int max_nbubble, void *stream)
 {
-    if ((space != HP3D_H1 && space != HP3D_HCURL) || (precision != HP3D_FP64 && precision != HP3D_FP32))
+    if ((space != HP3D_H1 && space != HP3D_HCURL) || (precision != HP3D_FP64 && precision != HP3D_FP32 && precision != HP3D_BF16))
         return HP3D_E_ARG;
     if (nel < 0 || ngeo < 1 || ngeo > MAXN_H1 || max_ndof < 1 || max_nbubble < 0 || ld_k < 0)
         return HP3D_E_ARG;
@@ -1715,7 +1757,7 @@ int hp3d_gpu_celem_batch(int space, int precision, int nel, const int *norder, c
     a.bout = bout;
     a.status = status;
     hipStream_t st = (hipStream_t)stream;
-    if (max_ndof <= BR && max_nbubble <= NBF) {
+    if (max_ndof <= BR && max_nbubble <= NBF && precision != HP3D_BF16) {
         CondArgs cd{off_c, off_s, aii, bi, aschur, bschur};
         return launch_single(space, precision, a, cd, st);
     }

@@ -16,7 +16,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import (FP32, FP64, H1, HCURL, LOAD_CURL_SIN, LOAD_ONE, LOAD_SINSINSIN,  # noqa: F401
+from ._lib import (BF16, FP32, FP64, H1, HCURL, LOAD_CURL_SIN, LOAD_ONE, LOAD_SINSINSIN,  # noqa: F401
                    LOAD_ZERO, LOAD_CONV, Hp3dError, check)
 
 MAXP = 6

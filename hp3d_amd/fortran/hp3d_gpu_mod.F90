@@ -20,7 +20,7 @@ module hp3d_gpu_mod
   integer(c_int), parameter :: HP3D_E_ORDER = -1, HP3D_E_NOIFACE = -3, HP3D_E_FACTOR = -4
   integer(c_int), parameter :: HP3D_E_ARG = -5, HP3D_E_HIP = -6, HP3D_E_INIT = -7
   integer(c_int), parameter :: HP3D_H1 = 0, HP3D_HCURL = 1
-  integer(c_int), parameter :: HP3D_FP64 = 0, HP3D_FP32 = 1
+  integer(c_int), parameter :: HP3D_FP64 = 0, HP3D_FP32 = 1, HP3D_BF16 = 2
   integer(c_int), parameter :: HP3D_LOAD_ZERO = 0, HP3D_LOAD_SINSINSIN = 1, HP3D_LOAD_CONV = 2
   integer(c_int), parameter :: HP3D_LOAD_ONE = 3, HP3D_LOAD_POLY2 = 4, HP3D_LOAD_CURL_SIN = 10
   integer(c_int), parameter :: HP3D_FIELD_ERROR = 0, HP3D_FIELD_ADJOINT = 1, HP3D_FIELD_L2 = 2

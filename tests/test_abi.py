@@ -73,7 +73,7 @@ def test_entries_before_init_and_bad_arguments():
                                    None, None, None, 1, None) == _lib.E_ARG
     assert lib.hp3d_gpu_elem_batch(0, 0, 1, None, None, None, None, 8, None, 0, None, 0, None,
                                    None, None, None, 1, None) == _lib.E_ARG
-    assert lib.hp3d_gpu_stc_batch(0, 2, 1, None, None, 0, None, None, None, None, 0, None, None,
+    assert lib.hp3d_gpu_stc_batch(0, 3, 1, None, None, 0, None, None, None, None, 0, None, None,
                                   None, None, None, None, None, 8, 1, None) == _lib.E_ARG
     assert lib.hp3d_gpu_elem_batch(0, 0, 0, None, None, None, None, 8, None, 0, None, 0, None,
                                    None, None, None, 1, None) == 0  # empty batch

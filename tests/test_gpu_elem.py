@@ -179,3 +179,24 @@ def test_order_bins_mixed_mesh_and_graph_replay(gpu):
             assert torch.equal(out[key], eager[key]), (rep, key)
         batch.run(1)  # eager work between replays (different precision, own buffers)
         torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("space,p", [(0, 4), (1, 2)])
+def test_bf16_trial(gpu, space, p):
+    """BF16 operands, FP32 accumulation: K within bfloat16 accuracy of the
+    FP64 K (and clearly not FP32-exact), condensation still well defined."""
+    import torch
+
+    import hp3d_amd
+
+    m = mesh.structured((2, 2, 2), porder=p, perturb=0.1, mesh_seed=3, perm_seed=4)
+    batch = hp3d_amd.ElementBatch.from_mesh(space, m)
+    o16 = batch.run(hp3d_amd.BF16)
+    o64 = batch.run(hp3d_amd.FP64)
+    torch.cuda.synchronize()
+    batch.check_status()
+    errs = [rel(batch.k_of(o16, e), batch.k_of(o64, e)) for e in range(batch.nel)]
+    aerr = [rel(batch.aii_of(o16, e), batch.aii_of(o64, e)) for e in range(batch.nel)]
+    print("bf16 K error max %.2e median %.2e, Aii max %.2e" % (max(errs), np.median(errs), max(aerr)))
+    assert 1e-5 < max(errs) < 2e-2
+    assert max(aerr) < 5e-2
