@@ -223,6 +223,8 @@ def main():
                     help="time only the FP32 element+condensation launches (for counter runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run one device-to-device copy of MIB MiB and exit (counter scale)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner (default bin rules)")
     ap.add_argument("--fd", type=int, default=8,
                     help="cfg4: elements checked by central differences (step 1e-6)")
     args = ap.parse_args()
@@ -249,7 +251,12 @@ def main():
     hp3d_amd.init()
     m = mesh.config_mesh(args.workload)
     space = hp3d_amd.HCURL if args.workload == "cfg3" else hp3d_amd.H1
-    batch = hp3d_amd.ElementBatch.from_mesh(space, m)
+    rules, tuning = None, None
+    if not args.no_tune and not args.kernel_only:
+        from hp3d_amd import tune
+
+        rules, tuning = tune.autotune(space, m, hp3d_amd.FP32, reps=3)
+    batch = hp3d_amd.ElementBatch.from_mesh(space, m, rules=rules)
     out32 = batch.alloc_outputs(hp3d_amd.FP32)
     if args.kernel_only:
         s32, e32 = time_steps(batch, out32, hp3d_amd.FP32, args.steps, args.warmup)
@@ -311,7 +318,8 @@ def main():
         "interface_rows_per_s": float(batch.ni.sum()) / s32,
         "forms": {
             "batched_fp32": {"element_matrices_per_s": value, "ms_per_step": s32 * 1e3,
-                             "elem_kernel_ms": e32, "bins": [list(b) for b in batch.bins]},
+                             "elem_kernel_ms": e32,
+                             "bins": [list(b) + [p] for b, p in zip(batch.bins, batch.bin_paths)]},
             "batched_fp32_graph": {"element_matrices_per_s": nel / sg, "ms_per_step": sg * 1e3},
             "batched_fp64": {"element_matrices_per_s": nel / s64, "ms_per_step": s64 * 1e3,
                              "elem_kernel_ms": e64,
@@ -326,10 +334,16 @@ def main():
         },
         "fp32_vs_fp64_error": {"k_max": kmax, "k_median": kmed, "aii_max": amax,
                                "aii_median": amed},
+        "tuning": tuning,
     }
-    fused = batch.max_ndof <= 128 and batch.max_nbubble <= 32
-    kname = "elem_single_kernel<float> (element + condensation)" if fused else \
-        "elem_kernel<float> + stc_kernel<float>"
+    parts = []
+    for (lo, cnt, mnd, mnb), path in zip(batch.bins, batch.bin_paths):
+        if path == "fused":
+            parts.append("elem_single_kernel<float> (element + condensation, %d elements)" % cnt)
+        else:
+            ek = "elem_single_kernel<float>" if mnd <= 128 else "elem_kernel<float>"
+            parts.append("%s + stc_blocked_kernel<float> (%d elements)" % (ek, cnt))
+    kname = " ; ".join(parts)
     achieved = (fk + fc) / (e32 * 1e-3)
     line["roofline"] = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK["fp32"] / 1e12,
                         "unit": "TFLOP/s", "frac": achieved / PEAK["fp32"], "traffic": None,

@@ -29,6 +29,7 @@
 #include <string.h>
 
 #include <mutex>
+#include <type_traits>
 
 #include "../../include/hp3d_gpu.h"
 #include "hp3d_tables.h"
@@ -384,6 +385,37 @@ __device__ inline void write_sym_tile(V acc, int R, int C, int lim, S *out, size
 
 
 
+// FP32 accumulation with an error-free carry: after each chunk of A columns
+// the MFMA accumulator c is folded into a running FP32 sum h by TwoSum
+// (h' = fl(h + c), c' = (h + c) - h' exactly), so c only ever accumulates one
+// chunk's products on top of a residual and the FP32 rounding of the long
+// k-sum (k = 3Q or 6Q columns) no longer grows with Q.  The tile's value is
+// h + c, used in FP64 by the condensation and rounded once for FP32 output.
+// For FP64 accumulators the carry is absent (Hi is a placeholder).
+template <typename T, typename A> struct Carry {
+    typedef char hi_t;
+    __device__ static inline void zero(hi_t &) {}
+    __device__ static inline void fold(hi_t &, A &) {}
+    __device__ static inline double val(const hi_t &, const A &c, int j) { return (double)c[j]; }
+    __device__ static inline A sum(const hi_t &, const A &c) { return c; }
+};
+template <typename A> struct Carry<float, A> {
+    typedef A hi_t;
+    __device__ static inline void zero(hi_t &h) { h = A{0.f, 0.f, 0.f, 0.f}; }
+    __device__ static inline void fold(hi_t &h, A &c)
+    {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            float a = c[j], hh = h[j];
+            float t = hh + a, bp = t - hh;
+            c[j] = (hh - (t - bp)) + (a - bp);
+            h[j] = t;
+        }
+    }
+    __device__ static inline double val(const hi_t &h, const A &c, int j) { return (double)h[j] + (double)c[j]; }
+    __device__ static inline A sum(const hi_t &h, const A &c) { return h + c; }
+};
+
 // bfloat16 operands for the BF16 trial: round to nearest even, 4 values
 // packed per lane as v_mfma_f32_16x16x16_bf16 takes them (lane l supplies
 // A[l % 16][4 (l / 16) + i] and B[4 (l / 16) + i][l % 16], i = 0..3; the
@@ -405,7 +437,7 @@ __device__ inline s4_t bf4(const float *col, int ld, int off)
 // multiplied on v_mfma_f32_16x16x16_bf16 with FP32 accumulation (the BF16
 // trial); the A chunk is padded from KC to 32 columns with zeros.
 template <typename T, int SPACE, bool BF = false>
-__global__ __launch_bounds__(WG) void elem_kernel(ElemArgs a)
+__global__ __launch_bounds__(WG, 2) void elem_kernel(ElemArgs a)
 {
     typedef Mfma<T> M;
     typedef typename M::acc_t acc_t;
@@ -453,9 +485,14 @@ __global__ __launch_bounds__(WG) void elem_kernel(ElemArgs a)
     const int tiA = diag ? wave : 2 * wave, tiB = diag ? 7 - wave : 2 * wave + 1;
     const int nrI = min(BR, n - I * BR), nrJ = min(BR, n - J * BR);
     const int offJ = diag ? 0 : BR;
+    typedef Carry<std::conditional_t<BF, double, T>, acc_t> CY; // no carry under bfloat16 rounding
     acc_t acc[16];
+    typename CY::hi_t hi[16];
 #pragma unroll
-    for (int t = 0; t < 16; ++t) acc[t] = acc_t{0, 0, 0, 0};
+    for (int t = 0; t < 16; ++t) {
+        acc[t] = acc_t{0, 0, 0, 0};
+        CY::zero(hi[t]);
+    }
     unsigned valid = 0;
 #pragma unroll
     for (int t = 0; t < 16; ++t) {
@@ -517,6 +554,8 @@ __global__ __launch_bounds__(WG) void elem_kernel(ElemArgs a)
                         if (valid & (1u << t)) acc[t] = M::mma(t < 8 ? aA : aB, b[t & 7], acc[t]);
                 }
             }
+#pragma unroll
+            for (int t = 0; t < 16; ++t) CY::fold(hi[t], acc[t]);
             __syncthreads();
         }
     }
@@ -531,7 +570,8 @@ __global__ __launch_bounds__(WG) void elem_kernel(ElemArgs a)
     for (int t = 0; t < 16; ++t) {
         if (!(valid & (1u << t))) continue;
         int ti = t < 8 ? tiA : tiB, tj = t & 7;
-        write_sym_tile<acc_t, T, M>(acc[t], I * BR + ti * 16, J * BR + tj * 16, n, kout, ld, stage, lane);
+        write_sym_tile<acc_t, T, M>(CY::sum(hi[t], acc[t]), I * BR + ti * 16, J * BR + tj * 16, n, kout, ld,
+                                    stage, lane);
     }
 }
 
@@ -614,9 +654,14 @@ __global__ __launch_bounds__(WG, 2) void elem_single_kernel(ElemArgs a, CondArgs
     int offS[NSLOT];
 #pragma unroll
     for (int s = 0; s < NSLOT; ++s) offS[s] = (s < nA ? tiA + s : s - 1) * 16 + (lane & 15);
+    typedef Carry<T, acc_t> CY;
     acc_t acc[NSLOT];
+    typename CY::hi_t hi[NSLOT];
 #pragma unroll
-    for (int s = 0; s < NSLOT; ++s) acc[s] = acc_t{0, 0, 0, 0};
+    for (int s = 0; s < NSLOT; ++s) {
+        acc[s] = acc_t{0, 0, 0, 0};
+        CY::zero(hi[s]);
+    }
     unsigned valid = 0;
 #pragma unroll
     for (int s = 0; s < NSLOT; ++s) {
@@ -663,6 +708,8 @@ __global__ __launch_bounds__(WG, 2) void elem_single_kernel(ElemArgs a, CondArgs
         for (int c = 0; c < nch; ++c) {
             mfma(ac + (c & 1) * BUF);
             if (c + 1 < nch) build(g0, (c + 1) * QC, gend, ac + ((c + 1) & 1) * BUF);
+#pragma unroll
+            for (int s = 0; s < NSLOT; ++s) CY::fold(hi[s], acc[s]);
             __syncthreads();
         }
     }
@@ -693,7 +740,7 @@ __global__ __launch_bounds__(WG, 2) void elem_single_kernel(ElemArgs a, CondArgs
         for (int s = 0; s < NSLOT; ++s) {
             if (!(valid & (1u << s))) continue;
             int ti = s < nA ? tiA : tiB, tj = s < nA ? tiA + s : s - 1;
-            write_sym_tile<acc_t, T, M>(acc[s], ti * 16, tj * 16, n, kout, ld, kstage, lane);
+            write_sym_tile<acc_t, T, M>(CY::sum(hi[s], acc[s]), ti * 16, tj * 16, n, kout, ld, kstage, lane);
         }
     };
     if (!cond) {
@@ -710,7 +757,7 @@ __global__ __launch_bounds__(WG, 2) void elem_single_kernel(ElemArgs a, CondArgs
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             int r = ti * 16 + M::row(lane, j), c = tj * 16 + (lane & 15);
-            double v = (double)acc[s][j];
+            double v = CY::val(hi[s], acc[s], j);
             if (r < n && c < n) {
                 if (c >= ni) kb[(c - ni) * LDS1 + r] = v;
                 if (r >= ni) kb[(r - ni) * LDS1 + c] = v;
@@ -800,7 +847,7 @@ __global__ __launch_bounds__(WG, 2) void elem_single_kernel(ElemArgs a, CondArgs
         if (R >= ni || C >= ni) continue;
         d4_t da; // the tile in the accumulator's layout, promoted to FP64
 #pragma unroll
-        for (int j = 0; j < 4; ++j) da[j] = (double)acc[s][j];
+        for (int j = 0; j < 4; ++j) da[j] = CY::val(hi[s], acc[s], j);
         d4_t up = d4_t{0, 0, 0, 0};
         for (int k0 = 0; k0 < nb; k0 += 4) {
             int k = k0 + (lane >> 4);

@@ -157,17 +157,23 @@ class ElementBatch:
     OpenMP element loop by one launch per bin): bin 0 holds the elements whose
     matrix fits one 128 x 128 block with at most 32 bubbles -- K stays in
     registers and condensation runs in the same kernel -- and bin 1 the rest
-    (block-tiled element kernel, then the condensation kernel).  Internally the
-    elements are stored bin by bin; every accessor takes the caller's element
-    index.  Outputs are packed: element i's N_i x N_i matrix starts at
+    (block-tiled element kernel, then the condensation kernel); `rules`
+    changes the bin limits and paths (hp3d_amd.tune picks them by timing).
+    Internally the elements are stored bin by bin; every accessor takes the
+    caller's element index.  Outputs are packed: element i's N_i x N_i matrix starts at
     off_k[i] (internal order) with leading dimension N_i.
     """
 
     FUSED_MAX_NDOF = 128
     FUSED_MAX_NBUBBLE = 32
+    # bin rules: an element goes to the first rule whose limits it meets; a
+    # "fused" bin runs the single-block kernel (K in registers, condensation in
+    # the same launch; needs N <= 128, nb <= 32), a "two_pass" bin the
+    # block-tiled element kernel and then the blocked condensation
+    DEFAULT_RULES = ({"max_ndof": 128, "max_nbubble": 32, "path": "fused"}, {"path": "two_pass"})
 
     def __init__(self, space, norder, nori_e, nori_f, xnod, coef, load=LOAD_SINSINSIN,
-                 device=None, binned=True):
+                 device=None, binned=True, rules=None):
         import torch
 
         _lib.init()
@@ -177,11 +183,20 @@ class ElementBatch:
         check_orders(norder)
         self.nel = norder.shape[0]
         n, nb = [np.asarray(v, dtype=np.int64) for v in ndof(space, norder)]
-        fused = (n <= self.FUSED_MAX_NDOF) & (nb <= self.FUSED_MAX_NBUBBLE)
-        if binned:
-            self.perm = np.concatenate([np.nonzero(fused)[0], np.nonzero(~fused)[0]])
-        else:
-            self.perm = np.arange(self.nel)
+        rules = list(self.DEFAULT_RULES if rules is None else rules)
+        if not binned:
+            rules = [{"path": "fused"}] if ((n <= self.FUSED_MAX_NDOF) &
+                                           (nb <= self.FUSED_MAX_NBUBBLE)).all() else [{"path": "two_pass"}]
+        which = np.full(self.nel, -1, np.int64)
+        for ri, r in enumerate(rules):
+            ok = (which < 0) & (n <= r.get("max_ndof", 10 ** 9)) & (nb <= r.get("max_nbubble", 10 ** 9))
+            if r["path"] == "fused":
+                ok &= (n <= self.FUSED_MAX_NDOF) & (nb <= self.FUSED_MAX_NBUBBLE)
+            which[ok] = ri
+        if (which < 0).any():
+            raise ValueError("bin rules leave elements unassigned (the last rule should be open)")
+        self.rules = rules
+        self.perm = np.argsort(which, kind="stable")
         self.iperm = np.empty_like(self.perm)
         self.iperm[self.perm] = np.arange(self.nel)
         P = self.perm
@@ -192,11 +207,14 @@ class ElementBatch:
         self.max_ndof = int(n.max()) if self.nel else 1
         self.max_nbubble = int(nb.max()) if self.nel else 0
         self.norder_h = norder[P]
-        nf = int(fused.sum()) if binned else 0
-        self.bins = []
-        for lo, hi in ((0, nf), (nf, self.nel)) if binned else ((0, self.nel),):
-            if hi > lo:
+        self.bins, self.bin_paths = [], []
+        ws = which[P]
+        for ri, r in enumerate(rules):
+            idx = np.nonzero(ws == ri)[0]
+            if idx.size:
+                lo, hi = int(idx[0]), int(idx[-1]) + 1
                 self.bins.append((lo, hi - lo, int(self._n[lo:hi].max()), int(self._nb[lo:hi].max())))
+                self.bin_paths.append(r["path"])
         dev = torch.device("cuda") if device is None else torch.device(device)
         self.device = dev
         t = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt).to(dev)
@@ -238,11 +256,11 @@ class ElementBatch:
         return at
 
     @classmethod
-    def from_mesh(cls, space, mesh, load=None, device=None):
+    def from_mesh(cls, space, mesh, load=None, device=None, rules=None):
         if load is None:
             load = LOAD_SINSINSIN if space == H1 else LOAD_CURL_SIN
         return cls(space, mesh.norder, mesh.nori_e, mesh.nori_f, mesh.xnod(), mesh.coef,
-                   load=load, device=device)
+                   load=load, device=device, rules=rules)
 
     def alloc_outputs(self, precision=FP64, want_k=True, want_stc=True):
         import torch
@@ -250,7 +268,9 @@ class ElementBatch:
         dt = torch.float64 if precision == FP64 else torch.float32
         out = {}
         dev = self.device
-        if want_k:
+        # two-pass bins (and the BF16 trial, which has no single-block form)
+        # condense from K in HBM
+        if want_k or (want_stc and ("two_pass" in self.bin_paths or precision == BF16)):
             out["k"] = torch.empty(self.size_k, dtype=dt, device=dev)
             out["b"] = torch.empty(self.size_v, dtype=torch.float64, device=dev)
         if want_stc:
@@ -260,6 +280,25 @@ class ElementBatch:
             out["bschur"] = torch.empty(self.size_v, dtype=torch.float64, device=dev)
         return out
 
+    def _element_bin(self, lib, out, precision, st, lo, cnt, mnd):
+        at = self._ptrs(lo)
+        rc = lib.hp3d_gpu_elem_batch(
+            self.space, precision, cnt, at(self.norder, 19), at(self.nori_e, 12),
+            at(self.nori_f, 6), at(self.xnod, 3 * self.ngeo), self.ngeo, at(self.coef, 1),
+            self.load, at(self.off_k, 1), 0, _tptr(out["k"]), at(self.off_v, 1),
+            _tptr(out["b"]), at(self.status, 1), mnd, ctypes.c_void_p(st))
+        check(rc, "hp3d_gpu_elem_batch")
+
+    def _condense_bin(self, lib, out, precision, st, lo, cnt, mnd, mnb):
+        at = self._ptrs(lo)
+        rc = lib.hp3d_gpu_stc_batch(
+            self.space, precision, cnt, at(self.norder, 19), at(self.off_k, 1), 0,
+            _tptr(out["k"]), at(self.off_v, 1), _tptr(out["b"]), at(self.off_c, 1), 0,
+            _tptr(out["aii"]), _tptr(out["bi"]), at(self.off_s, 1), _tptr(out["aschur"]),
+            _tptr(out["bschur"]), _tptr(self.workspace), at(self.status, 1), mnd, mnb,
+            ctypes.c_void_p(st))
+        check(rc, "hp3d_gpu_stc_batch")
+
     def element_matrices(self, out, precision=FP64, stream=None):
         """Launch the element kernel(s) into out['k'], out['b'] (no sync)."""
         import torch
@@ -267,13 +306,7 @@ class ElementBatch:
         lib = _lib.load()
         st = torch.cuda.current_stream().cuda_stream if stream is None else stream
         for lo, cnt, mnd, _mnb in self.bins:
-            at = self._ptrs(lo)
-            rc = lib.hp3d_gpu_elem_batch(
-                self.space, precision, cnt, at(self.norder, 19), at(self.nori_e, 12),
-                at(self.nori_f, 6), at(self.xnod, 3 * self.ngeo), self.ngeo, at(self.coef, 1),
-                self.load, at(self.off_k, 1), 0, _tptr(out["k"]), at(self.off_v, 1),
-                _tptr(out["b"]), at(self.status, 1), mnd, ctypes.c_void_p(st))
-            check(rc, "hp3d_gpu_elem_batch")
+            self._element_bin(lib, out, precision, st, lo, cnt, mnd)
 
     def condense(self, out, precision=FP64, stream=None):
         """Launch static condensation of out['k'], out['b'] (no sync)."""
@@ -282,24 +315,21 @@ class ElementBatch:
         lib = _lib.load()
         st = torch.cuda.current_stream().cuda_stream if stream is None else stream
         for lo, cnt, mnd, mnb in self.bins:
-            at = self._ptrs(lo)
-            rc = lib.hp3d_gpu_stc_batch(
-                self.space, precision, cnt, at(self.norder, 19), at(self.off_k, 1), 0,
-                _tptr(out["k"]), at(self.off_v, 1), _tptr(out["b"]), at(self.off_c, 1), 0,
-                _tptr(out["aii"]), _tptr(out["bi"]), at(self.off_s, 1), _tptr(out["aschur"]),
-                _tptr(out["bschur"]), _tptr(self.workspace), at(self.status, 1), mnd, mnb,
-                ctypes.c_void_p(st))
-            check(rc, "hp3d_gpu_stc_batch")
+            self._condense_bin(lib, out, precision, st, lo, cnt, mnd, mnb)
 
     def celem(self, out, precision=FP64, stream=None):
-        """Element matrices and condensation, one launch per bin where the
-        element fits one block (fused), else two (no sync).  out['k'] and
-        out['b'] may be absent when every bin is fused."""
+        """Element matrices and condensation: per bin one launch (fused bins)
+        or two (two-pass bins), no sync.  out['k'] and out['b'] may be absent
+        when every bin is fused."""
         import torch
 
         lib = _lib.load()
         st = torch.cuda.current_stream().cuda_stream if stream is None else stream
-        for lo, cnt, mnd, mnb in self.bins:
+        for (lo, cnt, mnd, mnb), path in zip(self.bins, self.bin_paths):
+            if path == "two_pass":
+                self._element_bin(lib, out, precision, st, lo, cnt, mnd)
+                self._condense_bin(lib, out, precision, st, lo, cnt, mnd, mnb)
+                continue
             at = self._ptrs(lo)
             rc = lib.hp3d_gpu_celem_batch(
                 self.space, precision, cnt, at(self.norder, 19), at(self.nori_e, 12),
@@ -326,7 +356,8 @@ class ElementBatch:
         return g
 
     def run(self, precision=FP64, want_k=True, want_stc=True, out=None, fused=True):
-        """Element matrices and (optionally) condensation; returns the outputs."""
+        """Element matrices and (optionally) condensation; returns the outputs.
+        fused=False runs every bin as two passes."""
         if out is None:
             out = self.alloc_outputs(precision, want_k or not fused, want_stc)
         if want_stc and fused:

@@ -1,0 +1,86 @@
+"""Autotuning of the batched element path.
+
+The tunable "tiles" of this path are the two ways an element's stiffness is
+tiled on the chip and the limits that route elements between them:
+
+* the single-block kernel: one workgroup holds the element's whole K
+  (up to 128 x 128) in MFMA accumulator registers, 9 tiles of 16 x 16 per
+  wave, and condenses the bubbles in the same launch;
+* the block-tiled path: one workgroup per 128 x 128 block of K (or the
+  single-block kernel without condensation for small N), K written to HBM,
+  then the blocked MFMA condensation.
+
+For a mesh, ``autotune`` times (HIP events on the launch stream) the full
+element + condensation step of the batch under each candidate rule set --
+fused-bin limits N <= {64, 96, 128} x nb <= {8, 16, 32}, and everything
+two-pass -- and returns the fastest rules with the table of timings.
+``ElementBatch(..., rules=best)`` then runs with them.  The OpenMP loop of
+the reference (mumps_sc.F90:224-342) has no counterpart: it calls one
+element at a time.
+"""
+import time
+
+import numpy as np
+
+from ._lib import FP32
+from .elem import ElementBatch, ndof
+
+
+def candidate_rules():
+    out = []
+    for nmax in (64, 96, 128):
+        for nbmax in (8, 16, 32):
+            out.append(({"max_ndof": nmax, "max_nbubble": nbmax, "path": "fused"},
+                        {"path": "two_pass"}))
+    out.append(({"path": "two_pass"},))
+    return out
+
+
+def _time_step(batch, precision, reps):
+    import torch
+
+    out = batch.alloc_outputs(precision, want_k=True, want_stc=True)
+    batch.celem(out, precision)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        batch.celem(out, precision)
+    e1.record()
+    torch.cuda.synchronize()
+    batch.check_status()
+    ms = e0.elapsed_time(e1) / reps
+    del out
+    return ms
+
+
+def autotune(space, mesh, precision=FP32, reps=3, load=None, candidates=None):
+    """Returns (best_rules, table) where table lists (rules, ms per step)."""
+    import torch
+
+    n, nb = ndof(space, mesh.norder)
+    eligible = (np.asarray(n) <= ElementBatch.FUSED_MAX_NDOF) & \
+        (np.asarray(nb) <= ElementBatch.FUSED_MAX_NBUBBLE)
+    cands = candidate_rules() if candidates is None else candidates
+    if not eligible.any():
+        # no element fits one block: a single path, nothing to choose
+        cands = [({"path": "two_pass"},)]
+    table = []
+    seen = set()
+    t0 = time.perf_counter()
+    for rules in cands:
+        batch = ElementBatch.from_mesh(space, mesh, load=load, rules=rules)
+        key = tuple((lo, cnt, p) for (lo, cnt, _, _), p in zip(batch.bins, batch.bin_paths))
+        if key in seen:          # same partition as an earlier candidate
+            del batch
+            continue
+        seen.add(key)
+        ms = _time_step(batch, precision, reps)
+        table.append({"rules": [dict(r) for r in rules], "ms": ms,
+                      "bins": [[cnt, mnd, mnb, p] for (_, cnt, mnd, mnb), p in
+                               zip(batch.bins, batch.bin_paths)]})
+        del batch
+        torch.cuda.empty_cache()
+    best = min(table, key=lambda r: r["ms"])
+    return tuple(best["rules"]), {"candidates": table, "best_ms": best["ms"],
+                                  "tuning_s": time.perf_counter() - t0}

@@ -100,7 +100,10 @@ def test_hcurl_fp64(gpu, p):
 @pytest.mark.parametrize("precision", [0, 1])
 def test_fused_matches_two_pass(gpu, precision):
     """The single-launch form (condensation on the register-resident K) and the
-    two-pass form (element kernel, then stc_kernel from HBM) agree."""
+    two-pass form (element kernel, then the blocked condensation from HBM)
+    agree.  In FP32 the fused form condenses the carried sum h + c in FP64 and
+    the two-pass form the K rounded once to FP32, so they agree to FP32
+    rounding; in FP64 to a few ulps."""
     import torch
 
     import hp3d_amd
@@ -114,14 +117,15 @@ def test_fused_matches_two_pass(gpu, precision):
     torch.cuda.synchronize()
     batch.check_status()
     assert "k" not in o3
+    t1, t2 = (1e-13, 1e-12) if precision == 0 else (1e-6, 1e-6)
     for e in range(batch.nel):
         for o in (o1, o3):
-            assert rel(batch.aii_of(o, e), batch.aii_of(o2, e)) < 1e-13
-            assert rel(batch.aschur_of(o, e), batch.aschur_of(o2, e)) < 1e-12
+            assert rel(batch.aii_of(o, e), batch.aii_of(o2, e)) < t1
+            assert rel(batch.aschur_of(o, e), batch.aschur_of(o2, e)) < t2
             nb = int(batch.nb[e])
-            assert rel(batch.vec_of(o, "bschur", e, nb), batch.vec_of(o2, "bschur", e, nb)) < 1e-12
+            assert rel(batch.vec_of(o, "bschur", e, nb), batch.vec_of(o2, "bschur", e, nb)) < t2
             ni = int(batch.ni[e])
-            assert rel(batch.vec_of(o, "bi", e, ni), batch.vec_of(o2, "bi", e, ni)) < 1e-13
+            assert rel(batch.vec_of(o, "bi", e, ni), batch.vec_of(o2, "bi", e, ni)) < t1
         assert rel(batch.k_of(o1, e), batch.k_of(o2, e)) == 0.0
 
 
@@ -200,3 +204,31 @@ def test_bf16_trial(gpu, space, p):
     print("bf16 K error max %.2e median %.2e, Aii max %.2e" % (max(errs), np.median(errs), max(aerr)))
     assert 1e-5 < max(errs) < 2e-2
     assert max(aerr) < 5e-2
+
+
+def test_bin_rules_and_autotune(gpu):
+    """Every rule set the autotuner may pick computes the same element and
+    condensed matrices (FP64, to rounding); autotune returns one of them."""
+    import torch
+
+    import hp3d_amd
+    from hp3d_amd import tune
+
+    rng = np.random.default_rng(8)
+    porder = rng.choice(np.arange(1, 7), size=(4 * 4 * 3, 3), p=mesh.P_OUT)
+    m = mesh.structured((4, 4, 3), porder=porder, perturb=0.1, mesh_seed=2, perm_seed=3,
+                        coef=np.exp(rng.normal(size=48)))
+    ref_batch = hp3d_amd.ElementBatch.from_mesh(0, m)
+    ref = ref_batch.run(0)
+    for rules in tune.candidate_rules()[::3]:
+        b = hp3d_amd.ElementBatch.from_mesh(0, m, rules=rules)
+        o = b.run(0)
+        torch.cuda.synchronize()
+        b.check_status()
+        for e in range(m.nel):
+            assert rel(b.k_of(o, e), ref_batch.k_of(ref, e)) < 1e-13
+            assert rel(b.aii_of(o, e), ref_batch.aii_of(ref, e)) < 1e-12
+            assert rel(b.aschur_of(o, e), ref_batch.aschur_of(ref, e)) < 1e-11
+    best, info = tune.autotune(0, m, hp3d_amd.FP32, reps=2)
+    assert any(tuple(c["rules"]) == best for c in info["candidates"])
+    assert info["best_ms"] == min(c["ms"] for c in info["candidates"])
