@@ -195,6 +195,7 @@ def adjoint_leg(m, nfd):
             g64 = g
             fd_err = []
             x0 = st["x"]
+            prob.rtol = 1e-14   # J(theta +- h) must resolve differences of ~h * g
             for e in np.random.default_rng(4004).choice(m.nel, nfd, replace=False):
                 tp, tm = theta.copy(), theta.copy()
                 tp[e] += 1e-6

@@ -385,10 +385,10 @@ __device__ inline void write_sym_tile(V acc, int R, int C, int lim, S *out, size
 
 
 
-// FP32 accumulation with an error-free carry: after each chunk of A columns
-// the MFMA accumulator c is folded into a running FP32 sum h by TwoSum
-// (h' = fl(h + c), c' = (h + c) - h' exactly), so c only ever accumulates one
-// chunk's products on top of a residual and the FP32 rounding of the long
+// FP32 accumulation with an error-free carry: every ~96 A columns the MFMA
+// accumulator c is folded into a running FP32 sum h by TwoSum
+// (h' = fl(h + c), c' = (h + c) - h' exactly), so c only ever accumulates a
+// few chunks' products on top of a residual and the FP32 rounding of the long
 // k-sum (k = 3Q or 6Q columns) no longer grows with Q.  The tile's value is
 // h + c, used in FP64 by the condensation and rounded once for FP32 output.
 // For FP64 accumulators the carry is absent (Hi is a placeholder).
@@ -503,6 +503,7 @@ __global__ __launch_bounds__(WG, 2) void elem_kernel(ElemArgs a)
     const int nrows = diag ? BR : 2 * BR;
     const bool do_load = diag && a.bout != nullptr;
     double bsum = 0.0;
+    int nfold = 0;
 
     for (int g0 = 0; g0 < nint; g0 += GQ) {
         for (int lq = tid; lq < GQ && g0 + lq < nint; lq += WG)
@@ -554,8 +555,10 @@ __global__ __launch_bounds__(WG, 2) void elem_kernel(ElemArgs a)
                         if (valid & (1u << t)) acc[t] = M::mma(t < 8 ? aA : aB, b[t & 7], acc[t]);
                 }
             }
+            if ((++nfold & 3) == 0) { // every 4 chunks of KC = 24 columns
 #pragma unroll
-            for (int t = 0; t < 16; ++t) CY::fold(hi[t], acc[t]);
+                for (int t = 0; t < 16; ++t) CY::fold(hi[t], acc[t]);
+            }
             __syncthreads();
         }
     }
@@ -696,6 +699,9 @@ __global__ __launch_bounds__(WG, 2) void elem_single_kernel(ElemArgs a, CondArgs
         }
     };
 
+    // fold the FP32 accumulators every 96 columns (2 chunks)
+    constexpr int FOLD_EVERY = KCS >= 48 ? 2 : 4;
+    int nfold = 0;
     for (int g0 = 0; g0 < nint; g0 += GQ) {
         const int gend = min(GQ, nint - g0);
         if (tid < gend && point_geometry<SPACE>(es, g0 + tid, tv, td, dgeo, xn, a.ngeo, coef, a.load,
@@ -708,8 +714,10 @@ __global__ __launch_bounds__(WG, 2) void elem_single_kernel(ElemArgs a, CondArgs
         for (int c = 0; c < nch; ++c) {
             mfma(ac + (c & 1) * BUF);
             if (c + 1 < nch) build(g0, (c + 1) * QC, gend, ac + ((c + 1) & 1) * BUF);
+            if ((++nfold & (FOLD_EVERY - 1)) == 0) {
 #pragma unroll
-            for (int s = 0; s < NSLOT; ++s) CY::fold(hi[s], acc[s]);
+                for (int s = 0; s < NSLOT; ++s) CY::fold(hi[s], acc[s]);
+            }
             __syncthreads();
         }
     }
