@@ -54,44 +54,60 @@ __constant__ double c_gw[10][10] = HP3D_GAUSS_W;
 
 // ------------------------------------------------------------------ helpers
 
-// 1D factor tables for one direction at the n Gauss points (values, d/dxi)
+// 1D factor tables at the Gauss points of the three directions (values and
+// d/dxi), one work item per (direction, point, flip) so that one wave fills
+// all of them: 2 + 2 (MAXP - 1) + 2 MAXP entries per (direction, point)
+__device__ __forceinline__ void fill_table_item(int d, int q, int flip, int n, double (*tv)[NTAB * NPT],
+                                                double (*td)[NTAB * NPT])
+{
+    double x = 0.5 * (1.0 + c_gx[n - 1][q]);
+    if (flip == 0) {
+        tv[d][q * NTAB + 0] = 1.0 - x;
+        td[d][q * NTAB + 0] = -1.0;
+        tv[d][q * NTAB + 1] = x;
+        td[d][q * NTAB + 1] = 1.0;
+    }
+    // integrated Legendre phi_i(s) = (P_i(2s-1) - P_{i-2}(2s-1)) / (4i-2),
+    // d phi_i / ds = P_{i-1}(2s-1)   (PolyILegendre, Idec = .true.)
+    double s = flip ? 1.0 - x : x;
+    double y = 2.0 * s - 1.0, p[MAXP + 1];
+    p[0] = 1.0;
+    p[1] = y;
+    for (int i = 2; i <= MAXP; ++i)
+        p[i] = ((double)(2 * i - 1) * y * p[i - 1] - (double)(i - 1) * p[i - 2]) / (double)i;
+    for (int i = 2; i <= MAXP; ++i) {
+        tv[d][q * NTAB + t_phi(i, flip)] = (p[i] - p[i - 2]) / (double)(4 * i - 2);
+        td[d][q * NTAB + t_phi(i, flip)] = flip ? -p[i - 1] : p[i - 1];
+    }
+    // scaled Legendre P_i(s; t) with t = s0 + s1 evaluated as the reference
+    // does (HomLegendre of the oriented pair)
+    double t = (1.0 - x) + x;
+    double yy = 2.0 * s - t, tt = t * t, r[MAXP];
+    r[0] = 1.0;
+    r[1] = yy;
+    for (int i = 2; i < MAXP; ++i)
+        r[i] = ((double)(2 * i - 1) * yy * r[i - 1] - (double)(i - 1) * tt * r[i - 2]) / (double)i;
+    for (int i = 0; i < MAXP; ++i) {
+        tv[d][q * NTAB + t_leg(i, flip)] = r[i];
+        td[d][q * NTAB + t_leg(i, flip)] = 0.0;
+    }
+}
+
+// all three directions' tables; `lane` runs over one wave (64 >= 3 x NPT x 2)
+__device__ __forceinline__ void fill_tables3(const int *nq, int lane, double (*tv)[NTAB * NPT],
+                                             double (*td)[NTAB * NPT])
+{
+    int d = lane / (2 * NPT), q = (lane / 2) % NPT, flip = lane & 1;
+    if (d < 3 && q < nq[d]) fill_table_item(d, q, flip, nq[d], tv, td);
+}
+
+// the same, one direction per wave (kernels that fill them with waves 0-2)
 __device__ __forceinline__ void fill_tables(int n, double *tv, double *td)
 {
-    // tv/td laid out point-major, [NPT][NTAB]: a wave's lanes (rows with
-    // different table entries at the same point) read consecutive words
-    for (int q = threadIdx.x % 32; q < n; q += 32) {
-        double x = 0.5 * (1.0 + c_gx[n - 1][q]);
-        tv[q * NTAB + 0] = 1.0 - x;
-        td[q * NTAB + 0] = -1.0;
-        tv[q * NTAB + 1] = x;
-        td[q * NTAB + 1] = 1.0;
-        for (int flip = 0; flip < 2; ++flip) {
-            // integrated Legendre phi_i(s) = (P_i(2s-1) - P_{i-2}(2s-1)) / (4i-2),
-            // d phi_i / ds = P_{i-1}(2s-1)   (PolyILegendre, Idec = .true.)
-            double s = flip ? 1.0 - x : x;
-            double y = 2.0 * s - 1.0, p[MAXP + 1];
-            p[0] = 1.0;
-            p[1] = y;
-            for (int i = 2; i <= MAXP; ++i)
-                p[i] = ((double)(2 * i - 1) * y * p[i - 1] - (double)(i - 1) * p[i - 2]) / (double)i;
-            for (int i = 2; i <= MAXP; ++i) {
-                tv[q * NTAB + t_phi(i, flip)] = (p[i] - p[i - 2]) / (double)(4 * i - 2);
-                td[q * NTAB + t_phi(i, flip)] = flip ? -p[i - 1] : p[i - 1];
-            }
-            // scaled Legendre P_i(s; t) with t = s0 + s1 evaluated as the
-            // reference does (HomLegendre of the oriented pair)
-            double t = (1.0 - x) + x;
-            double yy = 2.0 * s - t, tt = t * t, r[MAXP];
-            r[0] = 1.0;
-            r[1] = yy;
-            for (int i = 2; i < MAXP; ++i)
-                r[i] = ((double)(2 * i - 1) * yy * r[i - 1] - (double)(i - 1) * tt * r[i - 2]) / (double)i;
-            for (int i = 0; i < MAXP; ++i) {
-                tv[q * NTAB + t_leg(i, flip)] = r[i];
-                td[q * NTAB + t_leg(i, flip)] = 0.0;
-            }
-        }
-    }
+    double (*v)[NTAB * NPT] = reinterpret_cast<double (*)[NTAB * NPT]>(tv);
+    double (*g)[NTAB * NPT] = reinterpret_cast<double (*)[NTAB * NPT]>(td);
+    int l = threadIdx.x & 63, q = l >> 1, flip = l & 1;
+    if (q < n) fill_table_item(0, q, flip, n, v, g);
 }
 
 struct Pt { int l1, l2, l3; };
@@ -635,9 +651,14 @@ __global__ __launch_bounds__(WG, 2) void elem_single_kernel(ElemArgs a, CondArgs
         return;
     }
     const int n = es.n, nint = es.nint;
-    if (wave < 3) fill_tables(es.nq[wave], tv[wave], td[wave]);
-    for (int r = tid; r < BR; r += WG) dsp[r] = r < n ? (SPACE == 0 ? desc_h1(r, es.o) : desc_hcurl(r, es.o)) : 0u;
-    for (int k = tid; k < a.ngeo; k += WG) dgeo[k] = desc_h1(k, es.o);
+    // wave 0 fills the three 1D tables while waves 1-3 build the descriptors
+    if (wave == 0) {
+        fill_tables3(es.nq, lane, tv, td);
+    } else {
+        for (int r = tid - 64; r < BR; r += WG - 64)
+            dsp[r] = r < n ? (SPACE == 0 ? desc_h1(r, es.o) : desc_hcurl(r, es.o)) : 0u;
+        for (int k = tid - 64; k < a.ngeo; k += WG - 64) dgeo[k] = desc_h1(k, es.o);
+    }
     for (int l = tid; l < nint; l += WG) {
         Pt p = point_ix(l, es.nq);
         pix[l] = (unsigned)p.l1 | ((unsigned)p.l2 << 4) | ((unsigned)p.l3 << 8);
@@ -689,13 +710,29 @@ __global__ __launch_bounds__(WG, 2) void elem_single_kernel(ElemArgs a, CondArgs
             for (int c = 0; c < NC; ++c) buf[(NC * q + c) * LDS1 + row] = (T)out[c];
         }
     };
+    // The slot mask is wave-uniform.  When some of this wave's tiles lie beyond
+    // N (N <= 112), a scalar branch skips their MFMAs; when all are live the
+    // unguarded loop keeps the MFMAs back to back.
+    const unsigned vmask = __builtin_amdgcn_readfirstlane(valid);
+    const bool all_live = vmask == (1u << NSLOT) - 1;
     auto mfma = [&](const T *buf) {
+        if (all_live) {
 #pragma unroll
-        for (int k0 = 0; k0 < KCS; k0 += 4) {
-            const T *col = buf + (k0 + (lane >> 4)) * LDS1;
-            const T aA = col[offA], aB = col[offB];
+            for (int k0 = 0; k0 < KCS; k0 += 4) {
+                const T *col = buf + (k0 + (lane >> 4)) * LDS1;
+                const T aA = col[offA], aB = col[offB];
 #pragma unroll
-            for (int s = 0; s < NSLOT; ++s) acc[s] = M::mma(s < nA ? aA : aB, col[offS[s]], acc[s]);
+                for (int s = 0; s < NSLOT; ++s) acc[s] = M::mma(s < nA ? aA : aB, col[offS[s]], acc[s]);
+            }
+        } else {
+#pragma unroll
+            for (int k0 = 0; k0 < KCS; k0 += 4) {
+                const T *col = buf + (k0 + (lane >> 4)) * LDS1;
+                const T aA = col[offA], aB = col[offB];
+#pragma unroll
+                for (int s = 0; s < NSLOT; ++s)
+                    if (vmask & (1u << s)) acc[s] = M::mma(s < nA ? aA : aB, col[offS[s]], acc[s]);
+            }
         }
     };
 
