@@ -936,16 +936,17 @@ __global__ __launch_bounds__(WG, 2) void elem_single_kernel(ElemArgs a, CondArgs
 //   A  row r of L and Y is column ni + r of K (K is symmetric: the reads and
 //      the writes are both contiguous); Y's column ni is the bubble load; the
 //      padding is an identity block in L and zeros in Y
-//   B  right-looking blocked Cholesky with panels of 16, carrying Y along:
-//      per panel the 16 x 16 diagonal block is factored in LDS, the panel
-//      below is solved (L21 = A21 L11^-T, one row per thread) and staged in
-//      LDS, the panel's rows of Y are solved (Y_k = L11^-1 Y_k, one column per
-//      thread), then L22 -= L21 L21^T and Y_rest -= L21 Y_k on FP64 MFMA
-//      (16 x 16 tiles, waves round robin) -- at the end Y = L^-1 [Abi | Bb]
-//   C  Aii = Kii - Y^T Y and Bi = Bi - Y^T y_b (upper tiles of Y^T Y on MFMA,
-//      operands straight from the slab, both triangles written coalesced)
-//   D  X = L^-T Y by blocked backward substitution (panel solve per column,
-//      then Y_above -= L_panel^T X_panel on MFMA)
+//   B  right-looking blocked Cholesky of L with panels of 16: wave 0 factors
+//      the diagonal block in LDS, the panel below is solved one row per
+//      thread and staged in LDS, L22 -= L21 L21^T on FP64 MFMA
+//   C  Y = L^-1 [Abi | Bb], left-looking: per panel the row block of L is
+//      staged in LDS, each 16 x 16 tile of Y_k gathers L[k, 0:k] Y[0:k] on
+//      MFMA and is solved against L_kk by one lane per column; every panel of
+//      Y is written once (a right-looking update would rewrite Y per panel)
+//   D  Aii = Kii - Y^T Y and Bi = Bi - Y^T y_b: upper tiles of Y^T Y in 2 x 2
+//      blocks on MFMA, operands straight from the slab, both triangles
+//      written coalesced
+//   C' X = L^-T Y by the same left-looking scheme backwards, in place
 //   E  ASchur = X[:, 0:ni] (nb x ni, leading dimension nb), BSchur = X[:, ni]
 constexpr int SB = 16;
 __host__ __device__ inline int rnd16(int x) { return (x + 15) & ~15; }
@@ -1038,38 +1039,40 @@ __global__ __launch_bounds__(WG) void stc_blocked_kernel(StcArgs a)
     }
     __syncthreads();
 
-    // B: factor and forward substitution
+    // B: right-looking blocked Cholesky of Abb (L only)
     const int np = nbp / SB;
     const int nyc = ldy / SB;
     for (int p = 0; p < np; ++p) {
         const int k0 = SB * p;
-        {
-            int i = tid >> 4, j = tid & 15;
-            l11[i * 17 + j] = L[(size_t)(k0 + i) * nbp + k0 + j];
+        // diagonal block: factored by wave 0 (wave-level ordering only)
+        if (wave == 0) {
+            for (int idx = lane; idx < SB * SB; idx += 64)
+                l11[(idx >> 4) * 17 + (idx & 15)] = L[(size_t)(k0 + (idx >> 4)) * nbp + k0 + (idx & 15)];
+            wave_sync_lds();
+            for (int j = 0; j < SB; ++j) {
+                if (lane == 0) {
+                    double d = l11[j * 17 + j];
+                    if (!(d > 0.0)) {
+                        s_fail = 1;
+                        d = 1.0;
+                    }
+                    l11[j * 17 + j] = sqrt(d);
+                }
+                wave_sync_lds();
+                if (lane > j && lane < SB) l11[lane * 17 + j] /= l11[j * 17 + j];
+                wave_sync_lds();
+                for (int idx = lane; idx < SB * SB; idx += 64) {
+                    int i = idx >> 4, m = idx & 15;
+                    if (i > j && m > j && m <= i) l11[i * 17 + m] -= l11[i * 17 + j] * l11[m * 17 + j];
+                }
+                wave_sync_lds();
+            }
+            for (int idx = lane; idx < SB * SB; idx += 64) {
+                int i = idx >> 4, j = idx & 15;
+                if (j <= i) L[(size_t)(k0 + i) * nbp + k0 + j] = l11[i * 17 + j];
+            }
         }
         __syncthreads();
-        for (int j = 0; j < SB; ++j) {
-            if (tid == 0) {
-                double d = l11[j * 17 + j];
-                if (!(d > 0.0)) {
-                    s_fail = 1;
-                    d = 1.0;
-                }
-                l11[j * 17 + j] = sqrt(d);
-            }
-            __syncthreads();
-            if (tid > j && tid < SB) l11[tid * 17 + j] /= l11[j * 17 + j];
-            __syncthreads();
-            {
-                int i = tid >> 4, m = tid & 15;
-                if (i > j && m > j && m <= i) l11[i * 17 + m] -= l11[i * 17 + j] * l11[m * 17 + j];
-            }
-            __syncthreads();
-        }
-        {
-            int i = tid >> 4, j = tid & 15;
-            if (j <= i) L[(size_t)(k0 + i) * nbp + k0 + j] = l11[i * 17 + j];
-        }
         const int nr = nbp - k0 - SB;
         for (int t = tid; t < nr; t += WG) {
             // keeps the 136 L11 reads inside the loop (hoisted, they cost
@@ -1081,10 +1084,10 @@ __global__ __launch_bounds__(WG) void stc_blocked_kernel(StcArgs a)
             for (int j = 0; j < SB; ++j) x[j] = L[row + j];
 #pragma unroll
             for (int j = 0; j < SB; ++j) {
-                double s = x[j];
+                double sx = x[j];
 #pragma unroll
-                for (int m = 0; m < j; ++m) s -= x[m] * l11[j * 17 + m];
-                x[j] = s / l11[j * 17 + j];
+                for (int m = 0; m < j; ++m) sx -= x[m] * l11[j * 17 + m];
+                x[j] = sx / l11[j * 17 + j];
             }
 #pragma unroll
             for (int j = 0; j < SB; ++j) {
@@ -1092,114 +1095,160 @@ __global__ __launch_bounds__(WG) void stc_blocked_kernel(StcArgs a)
                 P[j * PLD + t] = x[j];
             }
         }
+        __syncthreads();
+        const int nt = nr / SB, nL = nt * (nt + 1) / 2;
+        for (int t = wave; t < nL; t += WG / 64) {
+            int I, J;
+            block_ij(t, J, I); // I >= J: lower tiles of the trailing block
+            d4_t acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int ks = 0; ks < SB; ks += 4) {
+                const int kk = ks + (lane >> 4);
+                acc = MD::mma(P[kk * PLD + SB * I + (lane & 15)], P[kk * PLD + SB * J + (lane & 15)], acc);
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                L[(size_t)(k0 + SB + SB * I + MD::row(lane, j)) * nbp + k0 + SB + SB * J + (lane & 15)] -= acc[j];
+        }
+        __syncthreads();
+    }
+
+    double *stw = stage + wave * 16 * 17;
+    // Y rows r0..r0+15 -= acc (a tile in the MFMA layout, columns c0..)
+    auto tile_sub = [&](const d4_t &acc, int r0, int c0) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) Y[(size_t)(r0 + MD::row(lane, j)) * ldy + c0 + (lane & 15)] -= acc[j];
+    };
+    // rows k0..k0+15 of every column of Y: solve with L_kk (forward) or
+    // L_kk^T (backward), one column per thread
+    auto panel_solve = [&](int k0, bool backward) {
         for (int c = tid; c < ldy; c += WG) {
             __asm__ volatile("" ::: "memory");
             double y[SB];
 #pragma unroll
             for (int j = 0; j < SB; ++j) y[j] = Y[(size_t)(k0 + j) * ldy + c];
+            if (!backward) {
 #pragma unroll
-            for (int j = 0; j < SB; ++j) {
-                double s = y[j];
+                for (int j = 0; j < SB; ++j) {
+                    double sy = y[j];
 #pragma unroll
-                for (int m = 0; m < j; ++m) s -= l11[j * 17 + m] * y[m];
-                y[j] = s / l11[j * 17 + j];
+                    for (int m = 0; m < j; ++m) sy -= l11[j * 17 + m] * y[m];
+                    y[j] = sy / l11[j * 17 + j];
+                }
+            } else {
+#pragma unroll
+                for (int j = SB - 1; j >= 0; --j) {
+                    double sy = y[j];
+#pragma unroll
+                    for (int m = j + 1; m < SB; ++m) sy -= l11[m * 17 + j] * y[m];
+                    y[j] = sy / l11[j * 17 + j];
+                }
             }
 #pragma unroll
             for (int j = 0; j < SB; ++j) Y[(size_t)(k0 + j) * ldy + c] = y[j];
         }
-        __syncthreads();
-        const int nt = nr / SB;
-        const int nL = nt * (nt + 1) / 2, ntile = nL + nt * nyc;
-        for (int t = wave; t < ntile; t += WG / 64) {
-            int I, J;
-            const bool isL = t < nL;
-            if (isL) {
-                block_ij(t, J, I); // I >= J: lower tiles of the trailing block
-            } else {
-                I = (t - nL) / nyc;
-                J = (t - nL) % nyc;
-            }
-            d4_t acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-            for (int ks = 0; ks < SB; ks += 4) {
-                const int kk = ks + (lane >> 4);
-                double av = P[kk * PLD + SB * I + (lane & 15)];
-                double bv = isL ? P[kk * PLD + SB * J + (lane & 15)]
-                                : Y[(size_t)(k0 + kk) * ldy + SB * J + (lane & 15)];
-                acc = MD::mma(av, bv, acc);
-            }
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int r = k0 + SB + SB * I + MD::row(lane, j);
-                if (isL) L[(size_t)r * nbp + k0 + SB + SB * J + (lane & 15)] -= acc[j];
-                else Y[(size_t)r * ldy + SB * J + (lane & 15)] -= acc[j];
-            }
-        }
-        __syncthreads();
-    }
+    };
 
-    // C: Aii = Kii - Y^T Y, Bi = Bi - Y^T y_b
-    {
-        const int nst = nyc * (nyc + 1) / 2;
-        for (int t = wave; t < nst; t += WG / 64) {
-            int I, J;
-            block_ij(t, I, J); // I <= J
-            d4_t acc = {0.0, 0.0, 0.0, 0.0};
-            for (int k = 0; k < nbp; k += 4) {
-                const double *yr = Y + (size_t)(k + (lane >> 4)) * ldy + (lane & 15);
-                acc = MD::mma(yr[SB * I], yr[SB * J], acc);
-            }
-            const int cj = SB * J + (lane & 15);
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int ri = SB * I + MD::row(lane, j);
-                if (cj == ni && ri < ni) bi[ri] = b[ri] - acc[j];
-                double kv = (ri < ni && cj < ni) ? (double)K[(size_t)ri * ldk + cj] : 0.0;
-                acc[j] = kv - acc[j];
-            }
-            write_sym_tile<d4_t, double, MD>(acc, SB * I, SB * J, ni, aii, ldc, stage + wave * 16 * 17, lane);
-        }
-    }
-    __syncthreads();
-
-    // D: X = L^-T Y
-    for (int p = np - 1; p >= 0; --p) {
+    // C: forward substitution Y = L^-1 [Abi | Bb], left-looking: panel k is
+    // Y_k = L_kk^-1 (C_k - L[k, 0:k] Y[0:k]), written once
+    for (int p = 0; p < np; ++p) {
         const int k0 = SB * p;
+        // L row panel (16 x k0) as MFMA A operand, column kk at P + 16 kk; and L_kk
+        for (int idx = tid; idx < SB * k0; idx += WG) {
+            const int r = idx / k0, c = idx % k0;
+            P[c * SB + r] = L[(size_t)(k0 + r) * nbp + c];
+        }
         {
             int i = tid >> 4, j = tid & 15;
             l11[i * 17 + j] = L[(size_t)(k0 + i) * nbp + k0 + j];
         }
         __syncthreads();
-        for (int c = tid; c < ldy; c += WG) {
-            __asm__ volatile("" ::: "memory");
-            double y[SB];
+        if (k0 > 0) {
+            for (int J = wave; J < nyc; J += WG / 64) {
+                d4_t acc = {0.0, 0.0, 0.0, 0.0};
+                for (int kk0 = 0; kk0 < k0; kk0 += 4) {
+                    const int kk = kk0 + (lane >> 4);
+                    acc = MD::mma(P[kk * SB + (lane & 15)], Y[(size_t)kk * ldy + SB * J + (lane & 15)], acc);
+                }
+                tile_sub(acc, k0, SB * J);
+            }
+            __syncthreads();
+        }
+        panel_solve(k0, false);
+        __syncthreads();
+    }
+
+    // D: Aii = Kii - Y^T Y, Bi = Bi - Y^T y_b: upper tiles in 2 x 2 blocks
+    // (four MFMAs per four operand loads), operands straight from the slab
+    {
+        const int nbt = (nyc + 1) / 2, ntask = nbt * (nbt + 1) / 2;
+        for (int t = wave; t < ntask; t += WG / 64) {
+            int BI, BJ;
+            block_ij(t, BI, BJ); // BI <= BJ
+            const bool i1 = 2 * BI + 1 < nyc, j1 = 2 * BJ + 1 < nyc;
+            d4_t acc[2][2];
 #pragma unroll
-            for (int j = 0; j < SB; ++j) y[j] = Y[(size_t)(k0 + j) * ldy + c];
+            for (int a0 = 0; a0 < 2; ++a0)
 #pragma unroll
-            for (int j = SB - 1; j >= 0; --j) {
-                double s = y[j];
-#pragma unroll
-                for (int m = j + 1; m < SB; ++m) s -= l11[m * 17 + j] * y[m];
-                y[j] = s / l11[j * 17 + j];
+                for (int b0 = 0; b0 < 2; ++b0) acc[a0][b0] = d4_t{0.0, 0.0, 0.0, 0.0};
+            for (int k = 0; k < nbp; k += 4) {
+                const double *yr = Y + (size_t)(k + (lane >> 4)) * ldy + (lane & 15);
+                const double av0 = yr[32 * BI], bv0 = yr[32 * BJ];
+                const double av1 = i1 ? yr[32 * BI + 16] : 0.0, bv1 = j1 ? yr[32 * BJ + 16] : 0.0;
+                acc[0][0] = MD::mma(av0, bv0, acc[0][0]);
+                acc[0][1] = MD::mma(av0, bv1, acc[0][1]);
+                acc[1][0] = MD::mma(av1, bv0, acc[1][0]);
+                acc[1][1] = MD::mma(av1, bv1, acc[1][1]);
             }
 #pragma unroll
-            for (int j = 0; j < SB; ++j) Y[(size_t)(k0 + j) * ldy + c] = y[j];
+            for (int a0 = 0; a0 < 2; ++a0)
+#pragma unroll
+                for (int b0 = 0; b0 < 2; ++b0) {
+                    const int I = 2 * BI + a0, J = 2 * BJ + b0;
+                    if (I >= nyc || J >= nyc || I > J) continue;
+                    d4_t v = acc[a0][b0];
+                    const int cj = SB * J + (lane & 15);
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const int ri = SB * I + MD::row(lane, j);
+                        if (cj == ni && ri < ni) bi[ri] = b[ri] - v[j];
+                        double kv = (ri < ni && cj < ni) ? (double)K[(size_t)ri * ldk + cj] : 0.0;
+                        v[j] = kv - v[j];
+                    }
+                    write_sym_tile<d4_t, double, MD>(v, SB * I, SB * J, ni, aii, ldc, stw, lane);
+                }
+        }
+    }
+    __syncthreads();
+
+    // back substitution X = L^-T Y, left-looking: panel k is
+    // X_k = L_kk^-T (Y_k - L[k+1:, k]^T X[k+1:]), in place over Y
+    for (int p = np - 1; p >= 0; --p) {
+        const int k0 = SB * p, rb = nbp - k0 - SB;
+        // L column panel below the diagonal block as MFMA A operand
+        // (A[i][kk] = L(k0 + 16 + kk, k0 + i)), and L_kk
+        for (int idx = tid; idx < SB * rb; idx += WG) {
+            const int kk = idx / SB, i = idx % SB;
+            P[kk * SB + i] = L[(size_t)(k0 + SB + kk) * nbp + k0 + i];
+        }
+        {
+            int i = tid >> 4, j = tid & 15;
+            l11[i * 17 + j] = L[(size_t)(k0 + i) * nbp + k0 + j];
         }
         __syncthreads();
-        const int ntile = p * nyc;
-        for (int t = wave; t < ntile; t += WG / 64) {
-            const int I = t / nyc, J = t % nyc;
-            d4_t acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-            for (int ks = 0; ks < SB; ks += 4) {
-                const int kk = k0 + ks + (lane >> 4);
-                acc = MD::mma(L[(size_t)kk * nbp + SB * I + (lane & 15)],
-                              Y[(size_t)kk * ldy + SB * J + (lane & 15)], acc);
+        if (rb > 0) {
+            for (int J = wave; J < nyc; J += WG / 64) {
+                d4_t acc = {0.0, 0.0, 0.0, 0.0};
+                for (int kk0 = 0; kk0 < rb; kk0 += 4) {
+                    const int kk = kk0 + (lane >> 4);
+                    acc = MD::mma(P[kk * SB + (lane & 15)], Y[(size_t)(k0 + SB + kk) * ldy + SB * J + (lane & 15)],
+                                  acc);
+                }
+                tile_sub(acc, k0, SB * J);
             }
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-                Y[(size_t)(SB * I + MD::row(lane, j)) * ldy + SB * J + (lane & 15)] -= acc[j];
+            __syncthreads();
         }
+        panel_solve(k0, true);
         __syncthreads();
     }
 
