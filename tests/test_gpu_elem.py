@@ -232,3 +232,25 @@ def test_bin_rules_and_autotune(gpu):
     best, info = tune.autotune(0, m, hp3d_amd.FP32, reps=2)
     assert any(tuple(c["rules"]) == best for c in info["candidates"])
     assert info["best_ms"] == min(c["ms"] for c in info["candidates"])
+
+
+@pytest.mark.parametrize("p", [4, 5, 6])
+def test_hcurl_large_fp64(gpu, p):
+    """H(curl) up to MAXP (N = 882, nb = 450): block-tiled element kernel and
+    the large-panel blocked condensation, against the oracle."""
+    m = mesh.structured((2, 1, 1), porder=p, perturb=0.1, mesh_seed=40 + p, perm_seed=50 + p,
+                        coef=np.array([1.5, 3.0]))
+    batch, out, ref = run_batch(m, 1, 0)
+    compare(batch, out, ref, TOL64, 1e-10)
+
+
+def test_hcurl_fp32_carry(gpu):
+    """FP32 contraction with the TwoSum carry on a long k-sum (H(curl) p = 4,
+    k = 6 x 125 = 750 columns): K and the condensed Aii within 1e-6 of the
+    oracle.  ASchur = Abb^-1 Abi inherits cond(Abb) times K's FP32 error and
+    is held to 1e-3, as in the H1 FP32 test."""
+    m = mesh.structured((2, 2, 1), porder=4, perturb=0.1, mesh_seed=7, perm_seed=8,
+                        coef=np.random.default_rng(4).uniform(1, 4, 4))
+    batch, out, ref = run_batch(m, 1, 1)
+    w = compare(batch, out, ref, TOL32, 1e-3)
+    assert w["k"] < 5e-7 and w["aii"] < 1e-6 and w["bi"] < 1e-6
