@@ -30,7 +30,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import FIELD_ERROR, H1, check
+from ._lib import FIELD_ERROR, H1, Hp3dError, check
 from .elem import ElementBatch
 from .mesh import EDGE_VERTS, FACE_VERTS
 
@@ -238,6 +238,8 @@ class PCGSolver:
     def __init__(self, asm, vals, rtol=1e-12, maxiter=20000, check_every=25):
         import torch
 
+        if asm.ndof >= 2 ** 31 - 1:
+            raise Hp3dError("PCG: %d unknowns exceed the int32 column indices" % asm.ndof)
         self.n = asm.ndof
         self.indptr = asm.indptr.contiguous()
         self.indices = asm.indices.to(torch.int32).contiguous()
@@ -250,6 +252,7 @@ class PCGSolver:
         self.dinv = 1.0 / diag
         self.rtol, self.maxiter, self.check_every = rtol, maxiter, check_every
         self.iterations = 0
+        self.relres = None
 
     def matvec(self, x, y):
         import torch
@@ -277,15 +280,19 @@ class PCGSolver:
         if bnorm == 0.0:
             return torch.zeros_like(b)
         it = 0
-        while it < self.maxiter:
+        while True:
             self.matvec(p, ap)
             alpha = rz / torch.dot(p, ap)
             x.add_(alpha * p)
             r.sub_(alpha * ap)
             it += 1
-            if it % self.check_every == 0:
-                if float(torch.linalg.vector_norm(r)) <= rtol * bnorm:
+            if it % self.check_every == 0 or it >= self.maxiter:
+                self.relres = float(torch.linalg.vector_norm(r)) / bnorm
+                if self.relres <= rtol:
                     break
+                if it >= self.maxiter:
+                    raise Hp3dError("PCG did not reach rtol %.1e in %d iterations (residual %.2e)"
+                                    % (rtol, it, self.relres))
             torch.mul(self.dinv, r, out=z)
             rz_new = torch.dot(r, z)
             p.mul_(rz_new / rz).add_(z)
@@ -315,7 +322,9 @@ def back_substitute(batch, out, u):
 def field(batch, mode, u, v=None, exact=0, with_r=False):
     """hp3d_gpu_field_batch over the batch (internal order).  Returns
     (out, r): out (nel, 2) for FIELD_ERROR else (nel,), r the element
-    vectors (FIELD_L2) or None."""
+    vectors (FIELD_L2) or None.  The kernel reports into batch.status
+    (negative Jacobian / invalid orders), replacing the element kernels'
+    codes."""
     import torch
 
     lib = _lib.load()
