@@ -943,7 +943,7 @@ __global__ __launch_bounds__(WG, 2) void elem_single_kernel(ElemArgs a, CondArgs
 //      staged in LDS, each 16 x 16 tile of Y_k gathers L[k, 0:k] Y[0:k] on
 //      MFMA and is solved against L_kk by one lane per column; every panel of
 //      Y is written once (a right-looking update would rewrite Y per panel)
-//   D  Aii = Kii - Y^T Y and Bi = Bi - Y^T y_b: upper tiles of Y^T Y in 2 x 2
+//   D  Aii = Kii - Y^T Y and Bi = Bi - Y^T y_b: upper tiles of Y^T Y in 4 x 2
 //      blocks on MFMA, operands straight from the slab, both triangles
 //      written coalesced
 //   C' X = L^-T Y by the same left-looking scheme backwards, in place
@@ -968,7 +968,7 @@ __host__ __device__ inline size_t stc_slab(int nbp_max, int ldy_max)
 }
 
 template <typename T, int PLD>
-__global__ __launch_bounds__(WG) void stc_blocked_kernel(StcArgs a)
+__global__ __launch_bounds__(WG, 2) void stc_blocked_kernel(StcArgs a)
 {
     typedef Mfma<double> MD;
     __shared__ double l11[SB * 17];
@@ -1029,7 +1029,8 @@ __global__ __launch_bounds__(WG) void stc_blocked_kernel(StcArgs a)
     // A: gather
     for (int r = wave; r < nbp; r += WG / 64) {
         const T *kc = K + (size_t)(ni + r) * ldk;
-        for (int c = lane; c < nbp; c += 64)
+        // lower triangle only: no phase reads L above the diagonal as data
+        for (int c = lane; c <= r; c += 64)
             L[(size_t)r * nbp + c] = r < nb ? (c < nb ? (double)kc[ni + c] : 0.0) : (c == r ? 1.0 : 0.0);
         for (int c = lane; c < ldy; c += 64) {
             double v = 0.0;
@@ -1178,33 +1179,38 @@ __global__ __launch_bounds__(WG) void stc_blocked_kernel(StcArgs a)
         __syncthreads();
     }
 
-    // D: Aii = Kii - Y^T Y, Bi = Bi - Y^T y_b: upper tiles in 2 x 2 blocks
-    // (four MFMAs per four operand loads), operands straight from the slab
+    // D: Aii = Kii - Y^T Y, Bi = Bi - Y^T y_b: upper tiles in 4 x 2 blocks
+    // (eight MFMAs per six operand loads), operands straight from the slab
     {
-        const int nbt = (nyc + 1) / 2, ntask = nbt * (nbt + 1) / 2;
-        for (int t = wave; t < ntask; t += WG / 64) {
-            int BI, BJ;
-            block_ij(t, BI, BJ); // BI <= BJ
-            const bool i1 = 2 * BI + 1 < nyc, j1 = 2 * BJ + 1 < nyc;
-            d4_t acc[2][2];
+        constexpr int BRI = 4, BRJ = 2;
+        const int nbi = (nyc + BRI - 1) / BRI, nbj = (nyc + BRJ - 1) / BRJ;
+        // block (BI, BJ) covers tile rows BRI*BI.., columns BRJ*BJ..; keep the
+        // blocks that hold an upper tile
+        for (int t = wave; t < nbi * nbj; t += WG / 64) {
+            const int BI = t / nbj, BJ = t % nbj;
+            if (BRI * BI > BRJ * BJ + BRJ - 1) continue;
+            d4_t acc[BRI][BRJ];
 #pragma unroll
-            for (int a0 = 0; a0 < 2; ++a0)
+            for (int a0 = 0; a0 < BRI; ++a0)
 #pragma unroll
-                for (int b0 = 0; b0 < 2; ++b0) acc[a0][b0] = d4_t{0.0, 0.0, 0.0, 0.0};
+                for (int b0 = 0; b0 < BRJ; ++b0) acc[a0][b0] = d4_t{0.0, 0.0, 0.0, 0.0};
             for (int k = 0; k < nbp; k += 4) {
                 const double *yr = Y + (size_t)(k + (lane >> 4)) * ldy + (lane & 15);
-                const double av0 = yr[32 * BI], bv0 = yr[32 * BJ];
-                const double av1 = i1 ? yr[32 * BI + 16] : 0.0, bv1 = j1 ? yr[32 * BJ + 16] : 0.0;
-                acc[0][0] = MD::mma(av0, bv0, acc[0][0]);
-                acc[0][1] = MD::mma(av0, bv1, acc[0][1]);
-                acc[1][0] = MD::mma(av1, bv0, acc[1][0]);
-                acc[1][1] = MD::mma(av1, bv1, acc[1][1]);
+                double av[BRI], bv[BRJ];
+#pragma unroll
+                for (int a0 = 0; a0 < BRI; ++a0) av[a0] = BRI * BI + a0 < nyc ? yr[SB * (BRI * BI + a0)] : 0.0;
+#pragma unroll
+                for (int b0 = 0; b0 < BRJ; ++b0) bv[b0] = BRJ * BJ + b0 < nyc ? yr[SB * (BRJ * BJ + b0)] : 0.0;
+#pragma unroll
+                for (int a0 = 0; a0 < BRI; ++a0)
+#pragma unroll
+                    for (int b0 = 0; b0 < BRJ; ++b0) acc[a0][b0] = MD::mma(av[a0], bv[b0], acc[a0][b0]);
             }
 #pragma unroll
-            for (int a0 = 0; a0 < 2; ++a0)
+            for (int a0 = 0; a0 < BRI; ++a0)
 #pragma unroll
-                for (int b0 = 0; b0 < 2; ++b0) {
-                    const int I = 2 * BI + a0, J = 2 * BJ + b0;
+                for (int b0 = 0; b0 < BRJ; ++b0) {
+                    const int I = BRI * BI + a0, J = BRJ * BJ + b0;
                     if (I >= nyc || J >= nyc || I > J) continue;
                     d4_t v = acc[a0][b0];
                     const int cj = SB * J + (lane & 15);
