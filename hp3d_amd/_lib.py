@@ -16,6 +16,7 @@ FP64, FP32, BF16 = 0, 1, 2
 LOAD_ZERO, LOAD_SINSINSIN, LOAD_CONV, LOAD_ONE, LOAD_CURL_SIN = 0, 1, 2, 3, 10
 LOAD_POLY2 = 4
 FIELD_ERROR, FIELD_ADJOINT, FIELD_L2 = 0, 1, 2
+OPT_PERSISTENT = 1
 
 OK = 0
 NEG_JACOBIAN = 1
@@ -30,6 +31,7 @@ SIGNATURES = {
     "hp3d_gpu_init": (_i, [_i, _i]),
     "hp3d_gpu_finalize": (_i, []),
     "hp3d_gpu_last_error": (ctypes.c_char_p, []),
+    "hp3d_gpu_set_option": (_i, [_i, _i]),
     "hp3d_gpu_ndof": (_i, [_i, _p, _p, _p]),
     "hp3d_gpu_elem": (_i, [_i, _p, _p, _p, _p, _i, _d, _i, _p, _i, _p]),
     "hp3d_gpu_stc_fwd": (_i, [_i, _i, _p, _i, _p, _p, _p]),

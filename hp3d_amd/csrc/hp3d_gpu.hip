@@ -28,6 +28,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <mutex>
 #include <type_traits>
 
@@ -617,7 +618,7 @@ template <typename T> struct SingleCfg {
 };
 
 template <typename T, int SPACE>
-__global__ __launch_bounds__(WG, 2) void elem_single_kernel(ElemArgs a, CondArgs cd)
+__device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs &cd, const int e)
 {
     typedef Mfma<T> M;
     typedef typename M::acc_t acc_t;
@@ -644,7 +645,6 @@ __global__ __launch_bounds__(WG, 2) void elem_single_kernel(ElemArgs a, CondArgs
     __shared__ int s_fail;
     T *ac = reinterpret_cast<T *>(smem);
 
-    const int e = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (!element_setup<SPACE>(a, e, es)) {
         if (tid == 0 && a.status) a.status[e] = HP3D_E_ORDER;
@@ -922,6 +922,25 @@ __global__ __launch_bounds__(WG, 2) void elem_single_kernel(ElemArgs a, CondArgs
     for (int idx = tid; idx < nb * ni; idx += WG) xs[idx] = kb[(idx % nb) * LDS1 + idx / nb];
     if (tid < nb) cd.bschur[a.off_v[e] + tid] = kb[tid * LDS1 + n];
     if (tid == 0 && a.status) a.status[e] = s_fail ? HP3D_E_FACTOR : (es.negjac ? HP3D_NEG_JACOBIAN : 0);
+}
+
+// one workgroup per element
+template <typename T, int SPACE>
+__global__ __launch_bounds__(WG, 2) void elem_single_kernel(ElemArgs a, CondArgs cd)
+{
+    single_element<T, SPACE>(a, cd, blockIdx.x);
+}
+
+// persistent form: a fixed grid (2 workgroups per CU) walks the elements with
+// a grid stride; the barrier after each element keeps the next element's
+// setup from overwriting LDS a slower wave still reads
+template <typename T, int SPACE>
+__global__ __launch_bounds__(WG, 2) void elem_single_persistent(ElemArgs a, CondArgs cd)
+{
+    for (int e = blockIdx.x; e < a.nel; e += gridDim.x) {
+        single_element<T, SPACE>(a, cd, e);
+        __syncthreads();
+    }
 }
 
 // ------------------------------------------------------------ condensation
@@ -1656,6 +1675,10 @@ static int hip_fail(hipError_t r, const char *what)
         if (_r != hipSuccess) return hip_fail(_r, #x); \
     } while (0)
 
+// launch options (hp3d_gpu_set_option): read when a launch is made, so a
+// captured graph keeps the form it was captured with
+static int g_persistent = 0;
+
 static int nblocks_for(int max_ndof)
 {
     int t = (max_ndof + BR - 1) / BR;
@@ -1665,13 +1688,28 @@ static int nblocks_for(int max_ndof)
 static int launch_single(int space, int precision, const ElemArgs &args, const CondArgs &cd,
                          hipStream_t st)
 {
-    dim3 grid(args.nel), block(WG);
-    if (space == HP3D_H1) {
-        if (precision == HP3D_FP64) hipLaunchKernelGGL((elem_single_kernel<double, 0>), grid, block, 0, st, args, cd);
-        else hipLaunchKernelGGL((elem_single_kernel<float, 0>), grid, block, 0, st, args, cd);
+    dim3 block(WG);
+    if (g_persistent) {
+        int dev = 0, ncu = 0;
+        HIPCHK(hipGetDevice(&dev));
+        HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        dim3 grid(std::min(args.nel, 2 * std::max(ncu, 1)));
+        if (space == HP3D_H1) {
+            if (precision == HP3D_FP64) hipLaunchKernelGGL((elem_single_persistent<double, 0>), grid, block, 0, st, args, cd);
+            else hipLaunchKernelGGL((elem_single_persistent<float, 0>), grid, block, 0, st, args, cd);
+        } else {
+            if (precision == HP3D_FP64) hipLaunchKernelGGL((elem_single_persistent<double, 1>), grid, block, 0, st, args, cd);
+            else hipLaunchKernelGGL((elem_single_persistent<float, 1>), grid, block, 0, st, args, cd);
+        }
     } else {
-        if (precision == HP3D_FP64) hipLaunchKernelGGL((elem_single_kernel<double, 1>), grid, block, 0, st, args, cd);
-        else hipLaunchKernelGGL((elem_single_kernel<float, 1>), grid, block, 0, st, args, cd);
+        dim3 grid(args.nel);
+        if (space == HP3D_H1) {
+            if (precision == HP3D_FP64) hipLaunchKernelGGL((elem_single_kernel<double, 0>), grid, block, 0, st, args, cd);
+            else hipLaunchKernelGGL((elem_single_kernel<float, 0>), grid, block, 0, st, args, cd);
+        } else {
+            if (precision == HP3D_FP64) hipLaunchKernelGGL((elem_single_kernel<double, 1>), grid, block, 0, st, args, cd);
+            else hipLaunchKernelGGL((elem_single_kernel<float, 1>), grid, block, 0, st, args, cd);
+        }
     }
     HIPCHK(hipGetLastError());
     return 0;
@@ -1715,6 +1753,14 @@ using namespace hp3d;
 extern "C" {
 
 const char *hp3d_gpu_last_error(void) { return g.err; }
+
+int hp3d_gpu_set_option(int option, int value)
+{
+    switch (option) {
+    case HP3D_OPT_PERSISTENT: g_persistent = value ? 1 : 0; return 0;
+    default: return HP3D_E_ARG;
+    }
+}
 
 int hp3d_gpu_ndof(int space, const int norder[19], int *nrdof, int *nrdofb)
 {

@@ -168,8 +168,9 @@ class ElementBatch:
     FUSED_MAX_NBUBBLE = 32
     # bin rules: an element goes to the first rule whose limits it meets; a
     # "fused" bin runs the single-block kernel (K in registers, condensation in
-    # the same launch; needs N <= 128, nb <= 32), a "two_pass" bin the
-    # block-tiled element kernel and then the blocked condensation
+    # the same launch; needs N <= 128, nb <= 32), as a persistent grid when the
+    # rule says "persistent": True; a "two_pass" bin the block-tiled element
+    # kernel and then the blocked condensation
     DEFAULT_RULES = ({"max_ndof": 128, "max_nbubble": 32, "path": "fused"}, {"path": "two_pass"})
 
     def __init__(self, space, norder, nori_e, nori_f, xnod, coef, load=LOAD_SINSINSIN,
@@ -207,7 +208,7 @@ class ElementBatch:
         self.max_ndof = int(n.max()) if self.nel else 1
         self.max_nbubble = int(nb.max()) if self.nel else 0
         self.norder_h = norder[P]
-        self.bins, self.bin_paths = [], []
+        self.bins, self.bin_paths, self.bin_persistent = [], [], []
         ws = which[P]
         for ri, r in enumerate(rules):
             idx = np.nonzero(ws == ri)[0]
@@ -215,6 +216,7 @@ class ElementBatch:
                 lo, hi = int(idx[0]), int(idx[-1]) + 1
                 self.bins.append((lo, hi - lo, int(self._n[lo:hi].max()), int(self._nb[lo:hi].max())))
                 self.bin_paths.append(r["path"])
+                self.bin_persistent.append(bool(r.get("persistent", False)))
         dev = torch.device("cuda") if device is None else torch.device(device)
         self.device = dev
         t = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt).to(dev)
@@ -325,11 +327,13 @@ class ElementBatch:
 
         lib = _lib.load()
         st = torch.cuda.current_stream().cuda_stream if stream is None else stream
-        for (lo, cnt, mnd, mnb), path in zip(self.bins, self.bin_paths):
+        for (lo, cnt, mnd, mnb), path, pers in zip(self.bins, self.bin_paths, self.bin_persistent):
             if path == "two_pass":
                 self._element_bin(lib, out, precision, st, lo, cnt, mnd)
                 self._condense_bin(lib, out, precision, st, lo, cnt, mnd, mnb)
                 continue
+            if pers:
+                check(lib.hp3d_gpu_set_option(_lib.OPT_PERSISTENT, 1), "hp3d_gpu_set_option")
             at = self._ptrs(lo)
             rc = lib.hp3d_gpu_celem_batch(
                 self.space, precision, cnt, at(self.norder, 19), at(self.nori_e, 12),
@@ -338,6 +342,8 @@ class ElementBatch:
                 _tptr(out.get("b")), at(self.off_c, 1), _tptr(out["aii"]), _tptr(out["bi"]),
                 at(self.off_s, 1), _tptr(out["aschur"]), _tptr(out["bschur"]),
                 _tptr(self.workspace), at(self.status, 1), mnd, mnb, ctypes.c_void_p(st))
+            if pers:
+                lib.hp3d_gpu_set_option(_lib.OPT_PERSISTENT, 0)
             check(rc, "hp3d_gpu_celem_batch")
 
     def capture(self, out, precision=FP64):

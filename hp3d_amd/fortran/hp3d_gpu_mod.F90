@@ -24,6 +24,7 @@ module hp3d_gpu_mod
   integer(c_int), parameter :: HP3D_LOAD_ZERO = 0, HP3D_LOAD_SINSINSIN = 1, HP3D_LOAD_CONV = 2
   integer(c_int), parameter :: HP3D_LOAD_ONE = 3, HP3D_LOAD_POLY2 = 4, HP3D_LOAD_CURL_SIN = 10
   integer(c_int), parameter :: HP3D_FIELD_ERROR = 0, HP3D_FIELD_ADJOINT = 1, HP3D_FIELD_L2 = 2
+  integer(c_int), parameter :: HP3D_OPT_PERSISTENT = 1
 
   interface
     integer(c_int) function hp3d_gpu_init(device, max_ndof) bind(C, name="hp3d_gpu_init")
@@ -37,6 +38,11 @@ module hp3d_gpu_mod
 
     type(c_ptr) function hp3d_gpu_last_error() bind(C, name="hp3d_gpu_last_error")
       import :: c_ptr
+    end function
+
+    integer(c_int) function hp3d_gpu_set_option(option, value) bind(C, name="hp3d_gpu_set_option")
+      import :: c_int
+      integer(c_int), value :: option, value
     end function
 
     integer(c_int) function hp3d_gpu_ndof(space, norder, nrdof, nrdofb) &

@@ -8,12 +8,15 @@ tiled on the chip and the limits that route elements between them:
   wave, and condenses the bubbles in the same launch;
 * the block-tiled path: one workgroup per 128 x 128 block of K (or the
   single-block kernel without condensation for small N), K written to HBM,
-  then the blocked MFMA condensation.
+  then the blocked MFMA condensation;
+* for the single-block kernel, one workgroup per element or a persistent
+  grid of 2 workgroups per CU walking the bin's elements.
 
 For a mesh, ``autotune`` times (HIP events on the launch stream) the full
 element + condensation step of the batch under each candidate rule set --
 fused-bin limits N <= {64, 96, 128} x nb <= {8, 16, 32}, and everything
-two-pass -- and returns the fastest rules with the table of timings.
+two-pass -- then the persistent form of the winner's fused bins, and
+returns the fastest rules with the table of timings.
 ``ElementBatch(..., rules=best)`` then runs with them.  The OpenMP loop of
 the reference (mumps_sc.F90:224-342) has no counterpart: it calls one
 element at a time.
@@ -82,5 +85,17 @@ def autotune(space, mesh, precision=FP32, reps=3, load=None, candidates=None):
         del batch
         torch.cuda.empty_cache()
     best = min(table, key=lambda r: r["ms"])
+    # second stage: the fused bins of the best rules as a persistent grid
+    if any(r["path"] == "fused" for r in best["rules"]):
+        rules = tuple(dict(r, persistent=True) if r["path"] == "fused" else dict(r)
+                      for r in best["rules"])
+        batch = ElementBatch.from_mesh(space, mesh, load=load, rules=rules)
+        ms = _time_step(batch, precision, reps)
+        table.append({"rules": [dict(r) for r in rules], "ms": ms,
+                      "bins": [[cnt, mnd, mnb, p] for (_, cnt, mnd, mnb), p in
+                               zip(batch.bins, batch.bin_paths)]})
+        del batch
+        torch.cuda.empty_cache()
+        best = min(table, key=lambda r: r["ms"])
     return tuple(best["rules"]), {"candidates": table, "best_ms": best["ms"],
                                   "tuning_s": time.perf_counter() - t0}

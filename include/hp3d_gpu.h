@@ -81,6 +81,15 @@ int hp3d_gpu_finalize(void);
 /* last HIP error string (for messages) */
 const char *hp3d_gpu_last_error(void);
 
+/* Process-wide launch options, read when a batched launch is made (a
+ * captured graph keeps the form it was captured with); not synchronised with
+ * launches in flight on other threads.  No counterpart in the reference.
+ *   HP3D_OPT_PERSISTENT  1: the single-block element kernel runs as a
+ *                        persistent grid (2 workgroups per CU walking the
+ *                        elements), 0: one workgroup per element (default) */
+#define HP3D_OPT_PERSISTENT 1
+int hp3d_gpu_set_option(int option, int value);
+
 /* ------------------------------------------------------------- counts */
 
 /* celndof / ndof_nod for a hexahedron (element/util/celndof.F90:24,

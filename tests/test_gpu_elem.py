@@ -254,3 +254,28 @@ def test_hcurl_fp32_carry(gpu):
     batch, out, ref = run_batch(m, 1, 1)
     w = compare(batch, out, ref, TOL32, 1e-3)
     assert w["k"] < 5e-7 and w["aii"] < 1e-6 and w["bi"] < 1e-6
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_persistent_single_block_kernel(gpu, precision):
+    """The persistent grid gives the same bits as one workgroup per element."""
+    import torch
+
+    import hp3d_amd
+
+    rng = np.random.default_rng(12)
+    porder = rng.choice(np.arange(1, 5), size=(6 * 5 * 4, 3), p=[0.1, 0.4, 0.3, 0.2])
+    m = mesh.structured((6, 5, 4), porder=porder, perturb=0.1, mesh_seed=5, perm_seed=6,
+                        coef=np.exp(rng.normal(size=120)))
+    plain = hp3d_amd.ElementBatch.from_mesh(0, m)
+    pers = hp3d_amd.ElementBatch.from_mesh(0, m, rules=(
+        {"max_ndof": 128, "max_nbubble": 32, "path": "fused", "persistent": True},
+        {"path": "two_pass"}))
+    assert pers.bin_persistent[0]
+    o1 = plain.run(precision)
+    o2 = pers.run(precision)
+    torch.cuda.synchronize()
+    plain.check_status()
+    pers.check_status()
+    for key in ("k", "b", "aii", "aschur"):
+        assert torch.equal(o1[key], o2[key]), key
