@@ -220,15 +220,16 @@ __device__ __forceinline__ void point_map(Pt p, const int *nq, const double (*tv
     det = jm[0][0] * jm[1][1] * jm[2][2] + jm[1][0] * jm[2][1] * jm[0][2] +
           jm[2][0] * jm[0][1] * jm[1][2] - jm[2][0] * jm[1][1] * jm[0][2] -
           jm[0][0] * jm[2][1] * jm[1][2] - jm[1][0] * jm[0][1] * jm[2][2];
-    inv[0][0] = (jm[1][1] * jm[2][2] - jm[2][1] * jm[1][2]) / det;
-    inv[1][0] = (-jm[1][0] * jm[2][2] + jm[2][0] * jm[1][2]) / det;
-    inv[2][0] = (jm[1][0] * jm[2][1] - jm[2][0] * jm[1][1]) / det;
-    inv[0][1] = (jm[2][1] * jm[0][2] - jm[0][1] * jm[2][2]) / det;
-    inv[1][1] = (jm[0][0] * jm[2][2] - jm[2][0] * jm[0][2]) / det;
-    inv[2][1] = (-jm[0][0] * jm[2][1] + jm[2][0] * jm[0][1]) / det;
-    inv[0][2] = (jm[0][1] * jm[1][2] - jm[1][1] * jm[0][2]) / det;
-    inv[1][2] = (-jm[0][0] * jm[1][2] + jm[1][0] * jm[0][2]) / det;
-    inv[2][2] = (jm[0][0] * jm[1][1] - jm[1][0] * jm[0][1]) / det;
+    const double rdet = 1.0 / det;
+    inv[0][0] = (jm[1][1] * jm[2][2] - jm[2][1] * jm[1][2]) * rdet;
+    inv[1][0] = (-jm[1][0] * jm[2][2] + jm[2][0] * jm[1][2]) * rdet;
+    inv[2][0] = (jm[1][0] * jm[2][1] - jm[2][0] * jm[1][1]) * rdet;
+    inv[0][1] = (jm[2][1] * jm[0][2] - jm[0][1] * jm[2][2]) * rdet;
+    inv[1][1] = (jm[0][0] * jm[2][2] - jm[2][0] * jm[0][2]) * rdet;
+    inv[2][1] = (-jm[0][0] * jm[2][1] + jm[2][0] * jm[0][1]) * rdet;
+    inv[0][2] = (jm[0][1] * jm[1][2] - jm[1][1] * jm[0][2]) * rdet;
+    inv[1][2] = (-jm[0][0] * jm[1][2] + jm[1][0] * jm[0][2]) * rdet;
+    inv[2][2] = (jm[0][0] * jm[1][1] - jm[1][0] * jm[0][1]) * rdet;
     w = (0.5 * c_gw[nq[0] - 1][p.l1]) * (0.5 * c_gw[nq[1] - 1][p.l2]) * (0.5 * c_gw[nq[2] - 1][p.l3]);
 }
 
@@ -342,10 +343,16 @@ __device__ __forceinline__ double load_entry(unsigned d, Pt p, const double (*tv
 template <int SPACE>
 __device__ bool element_setup(const ElemArgs &a, int e, ElemShared &es)
 {
-    if (threadIdx.x == 0) {
+    // the 37 header integers (norder, nori_e, nori_f) in one parallel load
+    __shared__ int hdr[40];
+    const int t = threadIdx.x;
+    if (t < 19) hdr[t] = a.norder[19 * (size_t)e + t];
+    else if (t < 31) hdr[t] = a.nori_e[12 * (size_t)e + (t - 19)];
+    else if (t < 37) hdr[t] = a.nori_f[6 * (size_t)e + (t - 31)];
+    __syncthreads();
+    if (t == 0) {
         ElemShared s;
-        s.ok = decode_orders(a.norder + 19 * (size_t)e, a.nori_e + 12 * (size_t)e,
-                             a.nori_f + 6 * (size_t)e, s.o);
+        s.ok = decode_orders(hdr, hdr + 19, hdr + 31, s.o);
         s.n = s.nb = s.nint = s.nh = 0;
         s.nq[0] = s.nq[1] = s.nq[2] = 1;
         if (s.ok) {
@@ -453,24 +460,35 @@ __device__ inline s4_t bf4(const float *col, int ld, int off)
 // BF (with T = float): the contraction's operands are rounded to bfloat16 and
 // multiplied on v_mfma_f32_16x16x16_bf16 with FP32 accumulation (the BF16
 // trial); the A chunk is padded from KC to 32 columns with zeros.
+// Each thread builds one row of the block's A (one row of an off-diagonal
+// block's two row sets, or one row of a diagonal block for half of the
+// chunk's points) with the row's descriptor read once; the A chunks are
+// double buffered (FP32 24 columns, FP64 12 columns per chunk), so a chunk's
+// build follows the MFMAs of the previous one with one barrier between chunks.
 template <typename T, int SPACE, bool BF = false>
 __global__ __launch_bounds__(WG, 2) void elem_kernel(ElemArgs a)
 {
     typedef Mfma<T> M;
     typedef typename M::acc_t acc_t;
-    constexpr int NC = SPACE == 0 ? 3 : 6;   // A columns per point
-    constexpr int QC = KC / NC;              // points per A chunk
-    constexpr int GQ = 64;                   // points per geometry pass
-    constexpr int GS = SPACE == 0 ? 10 : 21; // geometry doubles per point
-    constexpr int KCA = BF ? 32 : KC;        // allocated A columns
+    constexpr int NC = SPACE == 0 ? 3 : 6;             // A columns per point
+    constexpr int KCE = (BF || sizeof(T) == 4) ? KC : KC / 2; // A columns per chunk
+    constexpr int NBUF = BF ? 1 : 2;                   // A buffers
+    constexpr int QC = KCE / NC;                       // points per A chunk
+    constexpr int GQ = 64;                             // points per geometry pass
+    constexpr int GS = SPACE == 0 ? 10 : 21;           // geometry doubles per point
+    constexpr int KCA = BF ? 32 : KCE;                 // allocated A columns
+    constexpr int BUFE = KCA * LDA;                    // one A buffer, in T
     static_assert(!BF || sizeof(T) == 4, "bfloat16 operands come from an FP32 chunk");
+    static_assert(GQ % QC == 0 && QC % 2 == 0 || QC == 2, "a geometry pass holds whole chunks");
 
     __shared__ ElemShared es;
     __shared__ double tv[3][NTAB * NPT], td[3][NTAB * NPT];
     __shared__ unsigned dsp[2 * BR];   // descriptors of the block's rows
     __shared__ unsigned dgeo[MAXN_H1]; // H1 descriptors of the geometry dofs
+    __shared__ unsigned pix[MAXQ];     // packed 1D point indices
     __shared__ double geo[GQ * GS];
-    __shared__ T ac[KCA * LDA];
+    __shared__ double bv[BR];
+    __shared__ T ac[NBUF * BUFE];
 
     const int e = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -491,6 +509,10 @@ __global__ __launch_bounds__(WG, 2) void elem_kernel(ElemArgs a)
         dsp[r] = g < n ? (SPACE == 0 ? desc_h1(g, es.o) : desc_hcurl(g, es.o)) : 0u;
     }
     for (int k = tid; k < a.ngeo; k += WG) dgeo[k] = desc_h1(k, es.o);
+    for (int l = tid; l < nint; l += WG) {
+        Pt p = point_ix(l, es.nq);
+        pix[l] = (unsigned)p.l1 | ((unsigned)p.l2 << 4) | ((unsigned)p.l3 << 8);
+    }
     if (BF)
         for (int i = tid; i < (KCA - KC) * LDA; i += WG) ac[KC * LDA + i] = (T)0;
     __syncthreads();
@@ -517,70 +539,106 @@ __global__ __launch_bounds__(WG, 2) void elem_kernel(ElemArgs a)
         bool v = ti * 16 < nrI && tj * 16 < nrJ && (!diag || tj >= ti);
         valid |= (v ? 1u : 0u) << t;
     }
-    const int nrows = diag ? BR : 2 * BR;
+    const unsigned vmask = __builtin_amdgcn_readfirstlane(valid);
     const bool do_load = diag && a.bout != nullptr;
-    double bsum = 0.0;
-    int nfold = 0;
 
+    // this thread's row of A: row r of the block's row set (I rows, then J
+    // rows off the diagonal); a diagonal block's 128 rows are built by two
+    // threads each, one per half of the chunk's points
+    const int half = diag ? (tid >> 7) : 0;
+    const int row = diag ? (tid & (BR - 1)) : tid;
+    const int grow = row < BR ? I * BR + row : J * BR + row - BR;
+    const bool row_ok = grow < n;
+    const unsigned drow = dsp[row];
+    double bsum = 0.0;
+
+    auto build_pts = [&](auto npts, auto step, int g0, int l0, int gend, T *buf) {
+        constexpr int NP = decltype(npts)::value, ST = decltype(step)::value;
+#pragma unroll
+        for (int qq = 0; qq < NP; ++qq) {
+            const int q = ST * qq + half, lq = l0 + q;
+            double out[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) out[c] = 0.0;
+            if (row_ok && lq < gend) {
+                unsigned pk = pix[g0 + lq];
+                Pt p{(int)(pk & 15), (int)((pk >> 4) & 15), (int)(pk >> 8)};
+                bsum += a_entries_load<SPACE>(drow, p, tv, td, geo + lq * GS, out);
+            }
+#pragma unroll
+            for (int c = 0; c < NC; ++c) buf[(NC * q + c) * LDA + row] = (T)out[c];
+        }
+    };
+    auto build = [&](int g0, int l0, int gend, T *buf) {
+        if (diag)
+            build_pts(std::integral_constant<int, QC / 2>{}, std::integral_constant<int, 2>{}, g0, l0, gend, buf);
+        else
+            build_pts(std::integral_constant<int, QC>{}, std::integral_constant<int, 1>{}, g0, l0, gend, buf);
+    };
+    auto mfma = [&](const T *buf) {
+        if constexpr (BF) {
+#pragma unroll
+            for (int k0 = 0; k0 < KCA; k0 += 16) {
+                const float *col = (const float *)buf + (k0 + 4 * (lane >> 4)) * LDA;
+                s4_t aA = bf4(col, LDA, tiA * 16 + (lane & 15));
+                s4_t aB = bf4(col, LDA, tiB * 16 + (lane & 15));
+                s4_t b[8];
+#pragma unroll
+                for (int tj = 0; tj < 8; ++tj) b[tj] = bf4(col, LDA, offJ + tj * 16 + (lane & 15));
+#pragma unroll
+                for (int t = 0; t < 16; ++t)
+                    if (vmask & (1u << t))
+                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(t < 8 ? aA : aB, b[t & 7],
+                                                                          acc[t], 0, 0, 0);
+            }
+        } else {
+#pragma unroll
+            for (int k0 = 0; k0 < KCE; k0 += 4) {
+                const T *col = buf + (k0 + (lane >> 4)) * LDA;
+                T aA = col[tiA * 16 + (lane & 15)], aB = col[tiB * 16 + (lane & 15)];
+                T b[8];
+#pragma unroll
+                for (int tj = 0; tj < 8; ++tj) b[tj] = col[offJ + tj * 16 + (lane & 15)];
+#pragma unroll
+                for (int t = 0; t < 16; ++t)
+                    if (vmask & (1u << t)) acc[t] = M::mma(t < 8 ? aA : aB, b[t & 7], acc[t]);
+            }
+        }
+    };
+
+    int nfold = 0;
     for (int g0 = 0; g0 < nint; g0 += GQ) {
-        for (int lq = tid; lq < GQ && g0 + lq < nint; lq += WG)
-            if (point_geometry<SPACE>(es, g0 + lq, tv, td, dgeo, xn, a.ngeo, coef, a.load, geo + lq * GS))
-                es.negjac = 1;
-        __syncthreads();
         const int gend = min(GQ, nint - g0);
-        if (do_load && tid < BR && I * BR + tid < n)
-            for (int lq = 0; lq < gend; ++lq)
-                bsum += load_entry<SPACE>(dsp[tid], point_ix(g0 + lq, es.nq), tv, geo + lq * GS);
-        for (int l0 = 0; l0 < gend; l0 += QC) {
-            for (int it = tid; it < nrows * QC; it += WG) {
-                int r = it % nrows, q = it / nrows, lq = l0 + q;
-                int g = (r < BR ? I * BR + r : J * BR + r - BR);
-                double out[NC];
-#pragma unroll
-                for (int c = 0; c < NC; ++c) out[c] = 0.0;
-                if (g < n && lq < gend)
-                    a_entries<SPACE>(dsp[r], point_ix(g0 + lq, es.nq), tv, td, geo + lq * GS, out);
-#pragma unroll
-                for (int c = 0; c < NC; ++c) ac[(NC * q + c) * LDA + r] = (T)out[c];
+        if (tid < gend &&
+            point_geometry<SPACE>(es, g0 + tid, tv, td, dgeo, xn, a.ngeo, coef, a.load, geo + tid * GS))
+            es.negjac = 1;
+        __syncthreads();
+        const int nch = (gend + QC - 1) / QC;
+        build(g0, 0, gend, ac);
+        __syncthreads();
+        for (int c = 0; c < nch; ++c) {
+            mfma(ac + (NBUF == 2 ? (c & 1) * BUFE : 0));
+            if constexpr (NBUF == 2) {
+                if (c + 1 < nch) build(g0, (c + 1) * QC, gend, ac + ((c + 1) & 1) * BUFE);
             }
-            __syncthreads();
-            if constexpr (BF) {
-#pragma unroll
-                for (int k0 = 0; k0 < KCA; k0 += 16) {
-                    const float *col = (const float *)ac + (k0 + 4 * (lane >> 4)) * LDA;
-                    s4_t aA = bf4(col, LDA, tiA * 16 + (lane & 15));
-                    s4_t aB = bf4(col, LDA, tiB * 16 + (lane & 15));
-                    s4_t b[8];
-#pragma unroll
-                    for (int tj = 0; tj < 8; ++tj) b[tj] = bf4(col, LDA, offJ + tj * 16 + (lane & 15));
-#pragma unroll
-                    for (int t = 0; t < 16; ++t)
-                        if (valid & (1u << t))
-                            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(t < 8 ? aA : aB, b[t & 7],
-                                                                              acc[t], 0, 0, 0);
-                }
-            } else {
-#pragma unroll
-                for (int k0 = 0; k0 < KC; k0 += 4) {
-                    const T *col = ac + (k0 + (lane >> 4)) * LDA;
-                    T aA = col[tiA * 16 + (lane & 15)], aB = col[tiB * 16 + (lane & 15)];
-                    T b[8];
-#pragma unroll
-                    for (int tj = 0; tj < 8; ++tj) b[tj] = col[offJ + tj * 16 + (lane & 15)];
-#pragma unroll
-                    for (int t = 0; t < 16; ++t)
-                        if (valid & (1u << t)) acc[t] = M::mma(t < 8 ? aA : aB, b[t & 7], acc[t]);
-                }
-            }
-            if ((++nfold & 3) == 0) { // every 4 chunks of KC = 24 columns
+            if ((++nfold & 3) == 0) { // FP32: every 4 chunks of 24 columns
 #pragma unroll
                 for (int t = 0; t < 16; ++t) CY::fold(hi[t], acc[t]);
             }
             __syncthreads();
+            if constexpr (NBUF == 1) {
+                if (c + 1 < nch) {
+                    build(g0, (c + 1) * QC, gend, ac);
+                    __syncthreads();
+                }
+            }
         }
     }
 
-    if (do_load && tid < BR && I * BR + tid < n) a.bout[a.off_v[e] + I * BR + tid] = bsum;
+    // the load vector: the two halves of each diagonal row
+    if (do_load && half == 1) bv[row] = bsum;
+    __syncthreads();
+    if (do_load && half == 0 && row_ok) a.bout[a.off_v[e] + grow] = bsum + bv[row];
     if (tid == 0 && blockIdx.y == 0 && a.status) a.status[e] = es.negjac ? HP3D_NEG_JACOBIAN : 0;
 
     T *kout = (T *)a.kout + a.off_k[e];
@@ -608,6 +666,7 @@ __global__ __launch_bounds__(WG, 2) void elem_kernel(ElemArgs a)
 // the VALU/LDS work that builds chunk c+1, so the two overlap.
 constexpr int NBF = 32;      // largest bubble block condensed in the fused path
 constexpr int LDS1 = BR + 16; // LDS stride of the single-block kernel's A columns (see LDA)
+constexpr int XS_MAX = 27;   // geometry nodes staged in LDS by the single-block kernel
 struct CondArgs {
     const long long *off_c, *off_s;
     double *aii, *bi, *aschur, *bschur;
@@ -642,6 +701,7 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
     __shared__ unsigned pix[MAXQ];
     __shared__ double geo[GQ * GS];
     __shared__ double smem[SM_D];
+    __shared__ double xg[3 * XS_MAX];
     __shared__ int s_fail;
     T *ac = reinterpret_cast<T *>(smem);
 
@@ -663,10 +723,14 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
         Pt p = point_ix(l, es.nq);
         pix[l] = (unsigned)p.l1 | ((unsigned)p.l2 << 4) | ((unsigned)p.l3 << 8);
     }
+    // geometry nodes of up to 27-node maps staged in LDS
+    const double *xn = a.xnod + 3 * (size_t)a.ngeo * e;
+    const bool xs_ok = a.ngeo <= XS_MAX;
+    if (xs_ok)
+        for (int k = tid; k < 3 * a.ngeo; k += WG) xg[k] = xn[k];
     if (tid == 0) s_fail = 0;
     __syncthreads();
 
-    const double *xn = a.xnod + 3 * (size_t)a.ngeo * e;
     const double coef = a.coef[e];
     // this thread's row of A and its half of each chunk's points
     const int row = tid & (BR - 1), half = tid >> 7;
@@ -741,9 +805,13 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
     int nfold = 0;
     for (int g0 = 0; g0 < nint; g0 += GQ) {
         const int gend = min(GQ, nint - g0);
-        if (tid < gend && point_geometry<SPACE>(es, g0 + tid, tv, td, dgeo, xn, a.ngeo, coef, a.load,
-                                                 geo + tid * GS))
-            es.negjac = 1;
+        if (tid < gend) {
+            const bool neg = xs_ok ? point_geometry<SPACE>(es, g0 + tid, tv, td, dgeo, xg, a.ngeo, coef,
+                                                           a.load, geo + tid * GS)
+                                   : point_geometry<SPACE>(es, g0 + tid, tv, td, dgeo, xn, a.ngeo, coef,
+                                                           a.load, geo + tid * GS);
+            if (neg) es.negjac = 1;
+        }
         __syncthreads();
         const int nch = (gend + QC - 1) / QC;
         build(g0, 0, gend, ac);
