@@ -456,17 +456,22 @@ __device__ inline s4_t bf4(const float *col, int ld, int off)
     return s4_t{f2bf(col[off]), f2bf(col[ld + off]), f2bf(col[2 * ld + off]), f2bf(col[3 * ld + off])};
 }
 
-// General element kernel: one workgroup per (element, 128 x 128 block of K).
+// General element kernel: one workgroup of 8 waves per (element, 128 x 128
+// block of K).  An off-diagonal block's 64 tiles of 16 x 16 are 8 per wave
+// (wave w: tile row w); a diagonal block's 36 upper tiles are 5 or 4 per wave
+// (waves p and p + 4 share tile rows p and 7 - p, 9 tiles).  With 8 tiles a
+// wave fits in 128 VGPRs, so two workgroups give 4 waves per SIMD.
+// Each row of the block's A is built by 2 (off-diagonal) or 4 (diagonal)
+// threads, each for its share of the chunk's points, with the row's
+// descriptor read once; the A chunks are double buffered (FP32 24, FP64 12
+// columns per chunk), so a chunk's build follows the MFMAs of the previous
+// one with one barrier between chunks.
 // BF (with T = float): the contraction's operands are rounded to bfloat16 and
 // multiplied on v_mfma_f32_16x16x16_bf16 with FP32 accumulation (the BF16
 // trial); the A chunk is padded from KC to 32 columns with zeros.
-// Each thread builds one row of the block's A (one row of an off-diagonal
-// block's two row sets, or one row of a diagonal block for half of the
-// chunk's points) with the row's descriptor read once; the A chunks are
-// double buffered (FP32 24 columns, FP64 12 columns per chunk), so a chunk's
-// build follows the MFMAs of the previous one with one barrier between chunks.
+constexpr int WGE = 512; // threads per workgroup of the block-tiled kernel
 template <typename T, int SPACE, bool BF = false>
-__global__ __launch_bounds__(WG, 2) void elem_kernel(ElemArgs a)
+__global__ __launch_bounds__(WGE, 4) void elem_kernel(ElemArgs a)
 {
     typedef Mfma<T> M;
     typedef typename M::acc_t acc_t;
@@ -478,8 +483,9 @@ __global__ __launch_bounds__(WG, 2) void elem_kernel(ElemArgs a)
     constexpr int GS = SPACE == 0 ? 10 : 21;           // geometry doubles per point
     constexpr int KCA = BF ? 32 : KCE;                 // allocated A columns
     constexpr int BUFE = KCA * LDA;                    // one A buffer, in T
+    constexpr int NS = 8;                              // tile slots per wave
     static_assert(!BF || sizeof(T) == 4, "bfloat16 operands come from an FP32 chunk");
-    static_assert(GQ % QC == 0 && QC % 2 == 0 || QC == 2, "a geometry pass holds whole chunks");
+    static_assert(GQ % QC == 0 && QC % 2 == 0, "a geometry pass holds whole chunks");
 
     __shared__ ElemShared es;
     __shared__ double tv[3][NTAB * NPT], td[3][NTAB * NPT];
@@ -487,7 +493,7 @@ __global__ __launch_bounds__(WG, 2) void elem_kernel(ElemArgs a)
     __shared__ unsigned dgeo[MAXN_H1]; // H1 descriptors of the geometry dofs
     __shared__ unsigned pix[MAXQ];     // packed 1D point indices
     __shared__ double geo[GQ * GS];
-    __shared__ double bv[BR];
+    __shared__ double bv[3][BR];       // partial load sums of a diagonal block's rows
     __shared__ T ac[NBUF * BUFE];
 
     const int e = blockIdx.x;
@@ -504,59 +510,81 @@ __global__ __launch_bounds__(WG, 2) void elem_kernel(ElemArgs a)
     if (I * BR >= n || J * BR >= n) return; // block beyond this element's size
 
     if (wave < 3) fill_tables(es.nq[wave], tv[wave], td[wave]);
-    for (int r = tid; r < 2 * BR; r += WG) {
+    for (int r = tid; r < 2 * BR; r += WGE) {
         int g = (r < BR ? I * BR + r : J * BR + r - BR);
         dsp[r] = g < n ? (SPACE == 0 ? desc_h1(g, es.o) : desc_hcurl(g, es.o)) : 0u;
     }
-    for (int k = tid; k < a.ngeo; k += WG) dgeo[k] = desc_h1(k, es.o);
-    for (int l = tid; l < nint; l += WG) {
+    for (int k = tid; k < a.ngeo; k += WGE) dgeo[k] = desc_h1(k, es.o);
+    for (int l = tid; l < nint; l += WGE) {
         Pt p = point_ix(l, es.nq);
         pix[l] = (unsigned)p.l1 | ((unsigned)p.l2 << 4) | ((unsigned)p.l3 << 8);
     }
     if (BF)
-        for (int i = tid; i < (KCA - KC) * LDA; i += WG) ac[KC * LDA + i] = (T)0;
+        for (int i = tid; i < (KCA - KC) * LDA; i += WGE) ac[KC * LDA + i] = (T)0;
     __syncthreads();
 
     const double *xn = a.xnod + 3 * (size_t)a.ngeo * e;
     const double coef = a.coef[e];
 
-    // tiles of this wave: two tile rows (ti) x eight tile columns (tj)
-    const int tiA = diag ? wave : 2 * wave, tiB = diag ? 7 - wave : 2 * wave + 1;
+    // tile slots of this wave: slots s < nA in tile row tiA (tile columns
+    // tjA + s), the rest in tile row tiB (tile columns tjB + s - nA)
     const int nrI = min(BR, n - I * BR), nrJ = min(BR, n - J * BR);
     const int offJ = diag ? 0 : BR;
-    typedef Carry<std::conditional_t<BF, double, T>, acc_t> CY; // no carry under bfloat16 rounding
-    acc_t acc[16];
-    typename CY::hi_t hi[16];
-#pragma unroll
-    for (int t = 0; t < 16; ++t) {
-        acc[t] = acc_t{0, 0, 0, 0};
-        CY::zero(hi[t]);
+    int tiA, tiB, tjA, tjB, nA, nslot;
+    if (!diag) {
+        tiA = tiB = wave;
+        tjA = tjB = 0;
+        nA = nslot = NS;
+    } else if (wave < 4) {
+        tiA = wave;
+        tiB = 7 - wave;
+        tjA = wave;
+        tjB = 0;
+        nA = nslot = 5;
+    } else {
+        const int p = wave - 4;
+        tiA = p;
+        tiB = 7 - p;
+        tjA = p + 5;
+        tjB = 7 - p;
+        nA = 3 - p;
+        nslot = 4;
     }
+    int offS[NS];
     unsigned valid = 0;
 #pragma unroll
-    for (int t = 0; t < 16; ++t) {
-        int ti = t < 8 ? tiA : tiB, tj = t & 7;
-        bool v = ti * 16 < nrI && tj * 16 < nrJ && (!diag || tj >= ti);
-        valid |= (v ? 1u : 0u) << t;
+    for (int s = 0; s < NS; ++s) {
+        const int ti = s < nA ? tiA : tiB, tj = s < nA ? tjA + s : tjB + (s - nA);
+        offS[s] = offJ + tj * 16 + (lane & 15);
+        const bool v = s < nslot && ti * 16 < nrI && tj * 16 < nrJ;
+        valid |= (v ? 1u : 0u) << s;
     }
     const unsigned vmask = __builtin_amdgcn_readfirstlane(valid);
+    typedef Carry<std::conditional_t<BF, double, T>, acc_t> CY; // no carry under bfloat16 rounding
+    acc_t acc[NS];
+    typename CY::hi_t hi[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        acc[s] = acc_t{0, 0, 0, 0};
+        CY::zero(hi[s]);
+    }
     const bool do_load = diag && a.bout != nullptr;
 
-    // this thread's row of A: row r of the block's row set (I rows, then J
-    // rows off the diagonal); a diagonal block's 128 rows are built by two
-    // threads each, one per half of the chunk's points
-    const int half = diag ? (tid >> 7) : 0;
-    const int row = diag ? (tid & (BR - 1)) : tid;
+    // this thread's row of A (row r of the block's row set: I rows, then J
+    // rows off the diagonal) and its share sub of the chunk's points
+    const int nrows = diag ? BR : 2 * BR;
+    const int row = tid & (nrows - 1), sub = tid / nrows;
     const int grow = row < BR ? I * BR + row : J * BR + row - BR;
     const bool row_ok = grow < n;
     const unsigned drow = dsp[row];
     double bsum = 0.0;
 
-    auto build_pts = [&](auto npts, auto step, int g0, int l0, int gend, T *buf) {
-        constexpr int NP = decltype(npts)::value, ST = decltype(step)::value;
+    auto build_pts = [&](auto nshare, int g0, int l0, int gend, T *buf) {
+        constexpr int P = decltype(nshare)::value, NP = (QC + P - 1) / P;
 #pragma unroll
         for (int qq = 0; qq < NP; ++qq) {
-            const int q = ST * qq + half, lq = l0 + q;
+            const int q = P * qq + sub, lq = l0 + q;
+            if (QC % P != 0 && q >= QC) break;
             double out[NC];
 #pragma unroll
             for (int c = 0; c < NC; ++c) out[c] = 0.0;
@@ -570,10 +598,8 @@ __global__ __launch_bounds__(WG, 2) void elem_kernel(ElemArgs a)
         }
     };
     auto build = [&](int g0, int l0, int gend, T *buf) {
-        if (diag)
-            build_pts(std::integral_constant<int, QC / 2>{}, std::integral_constant<int, 2>{}, g0, l0, gend, buf);
-        else
-            build_pts(std::integral_constant<int, QC>{}, std::integral_constant<int, 1>{}, g0, l0, gend, buf);
+        if (diag) build_pts(std::integral_constant<int, WGE / BR>{}, g0, l0, gend, buf);
+        else build_pts(std::integral_constant<int, WGE / (2 * BR)>{}, g0, l0, gend, buf);
     };
     auto mfma = [&](const T *buf) {
         if constexpr (BF) {
@@ -582,26 +608,20 @@ __global__ __launch_bounds__(WG, 2) void elem_kernel(ElemArgs a)
                 const float *col = (const float *)buf + (k0 + 4 * (lane >> 4)) * LDA;
                 s4_t aA = bf4(col, LDA, tiA * 16 + (lane & 15));
                 s4_t aB = bf4(col, LDA, tiB * 16 + (lane & 15));
-                s4_t b[8];
 #pragma unroll
-                for (int tj = 0; tj < 8; ++tj) b[tj] = bf4(col, LDA, offJ + tj * 16 + (lane & 15));
-#pragma unroll
-                for (int t = 0; t < 16; ++t)
-                    if (vmask & (1u << t))
-                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(t < 8 ? aA : aB, b[t & 7],
-                                                                          acc[t], 0, 0, 0);
+                for (int s = 0; s < NS; ++s)
+                    if (vmask & (1u << s))
+                        acc[s] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(s < nA ? aA : aB, bf4(col, LDA, offS[s]),
+                                                                          acc[s], 0, 0, 0);
             }
         } else {
 #pragma unroll
             for (int k0 = 0; k0 < KCE; k0 += 4) {
                 const T *col = buf + (k0 + (lane >> 4)) * LDA;
-                T aA = col[tiA * 16 + (lane & 15)], aB = col[tiB * 16 + (lane & 15)];
-                T b[8];
+                const T aA = col[tiA * 16 + (lane & 15)], aB = col[tiB * 16 + (lane & 15)];
 #pragma unroll
-                for (int tj = 0; tj < 8; ++tj) b[tj] = col[offJ + tj * 16 + (lane & 15)];
-#pragma unroll
-                for (int t = 0; t < 16; ++t)
-                    if (vmask & (1u << t)) acc[t] = M::mma(t < 8 ? aA : aB, b[t & 7], acc[t]);
+                for (int s = 0; s < NS; ++s)
+                    if (vmask & (1u << s)) acc[s] = M::mma(s < nA ? aA : aB, col[offS[s]], acc[s]);
             }
         }
     };
@@ -623,7 +643,7 @@ __global__ __launch_bounds__(WG, 2) void elem_kernel(ElemArgs a)
             }
             if ((++nfold & 3) == 0) { // FP32: every 4 chunks of 24 columns
 #pragma unroll
-                for (int t = 0; t < 16; ++t) CY::fold(hi[t], acc[t]);
+                for (int s = 0; s < NS; ++s) CY::fold(hi[s], acc[s]);
             }
             __syncthreads();
             if constexpr (NBUF == 1) {
@@ -635,20 +655,20 @@ __global__ __launch_bounds__(WG, 2) void elem_kernel(ElemArgs a)
         }
     }
 
-    // the load vector: the two halves of each diagonal row
-    if (do_load && half == 1) bv[row] = bsum;
+    // the load vector: the four partial sums of each diagonal row
+    if (do_load && sub > 0) bv[sub - 1][row] = bsum;
     __syncthreads();
-    if (do_load && half == 0 && row_ok) a.bout[a.off_v[e] + grow] = bsum + bv[row];
+    if (do_load && sub == 0 && row_ok) a.bout[a.off_v[e] + grow] = bsum + bv[0][row] + bv[1][row] + bv[2][row];
     if (tid == 0 && blockIdx.y == 0 && a.status) a.status[e] = es.negjac ? HP3D_NEG_JACOBIAN : 0;
 
     T *kout = (T *)a.kout + a.off_k[e];
     const size_t ld = a.ld_k > 0 ? (size_t)a.ld_k : (size_t)n;
     T *stage = ac + wave * (16 * 17);
 #pragma unroll
-    for (int t = 0; t < 16; ++t) {
-        if (!(valid & (1u << t))) continue;
-        int ti = t < 8 ? tiA : tiB, tj = t & 7;
-        write_sym_tile<acc_t, T, M>(CY::sum(hi[t], acc[t]), I * BR + ti * 16, J * BR + tj * 16, n, kout, ld,
+    for (int s = 0; s < NS; ++s) {
+        if (!(valid & (1u << s))) continue;
+        const int ti = s < nA ? tiA : tiB, tj = s < nA ? tjA + s : tjB + (s - nA);
+        write_sym_tile<acc_t, T, M>(CY::sum(hi[s], acc[s]), I * BR + ti * 16, J * BR + tj * 16, n, kout, ld,
                                     stage, lane);
     }
 }
@@ -1790,7 +1810,7 @@ static int launch_elem(int space, int precision, const ElemArgs &args, int max_n
         CondArgs cd{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
         return launch_single(space, precision, args, cd, st);
     }
-    dim3 grid(args.nel, nblocks_for(max_ndof)), block(WG);
+    dim3 grid(args.nel, nblocks_for(max_ndof)), block(WGE);
     if (space == HP3D_H1) {
         if (precision == HP3D_FP64) hipLaunchKernelGGL((elem_kernel<double, 0>), grid, block, 0, st, args);
         else if (precision == HP3D_BF16) hipLaunchKernelGGL((elem_kernel<float, 0, true>), grid, block, 0, st, args);
