@@ -696,7 +696,7 @@ template <typename T> struct SingleCfg {
     static constexpr int KCS = sizeof(T) == 4 ? 48 : 24; // A columns per chunk
 };
 
-template <typename T, int SPACE>
+template <typename T, int SPACE, int R>
 __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs &cd, const int e)
 {
     typedef Mfma<T> M;
@@ -707,16 +707,22 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
     constexpr int QC = KCS / NC;               // points per chunk
     constexpr int GQ = 64;                     // points per geometry pass (multiple of QC)
     constexpr int GS = SPACE == 0 ? 10 : 21;
-    constexpr int NSLOT = 9;
-    constexpr int BUF = KCS * LDS1;            // one A buffer, in T
+    static_assert(R == BR || R == BR / 2, "128 or 64 rows");
+    constexpr int LD1 = R + 16;                // LDS stride of an A column and of a bubble row
+    constexpr int NBX = R == BR ? NBF : NBF / 2; // largest bubble block condensed here
+    constexpr int LW = NBX + 1;                // leading dimension of L_u^-1
+    constexpr int NSLOT = R == BR ? 9 : 3;     // tile slots per wave
+    constexpr int SH = WG / R;                 // threads per row, each with its share of the points
+    constexpr int UMAX = R == BR ? 5 : 2;      // right-hand-side tiles per wave in the condensation
+    constexpr int BUF = KCS * LD1;             // one A buffer, in T
     constexpr int AC_D = (2 * BUF * (int)sizeof(T) + 7) / 8;
-    constexpr int CW_D = NBF * LDS1 + NBF * 33 + NBF + LDS1 + 4 * 16 * 17; // output / condensation workspace
+    constexpr int CW_D = NBX * LD1 + NBX * LW + NBX + LD1 + 4 * 16 * 17 + (SH - 1) * R; // output / condensation workspace
     constexpr int SM_D = AC_D > CW_D ? AC_D : CW_D;
-    static_assert(GQ % QC == 0, "geometry passes hold whole chunks");
+    static_assert(GQ % QC == 0 && QC % SH == 0, "geometry passes hold whole chunks");
 
     __shared__ ElemShared es;
     __shared__ double tv[3][NTAB * NPT], td[3][NTAB * NPT];
-    __shared__ unsigned dsp[BR];
+    __shared__ unsigned dsp[R];
     __shared__ unsigned dgeo[MAXN_H1];
     __shared__ unsigned pix[MAXQ];
     __shared__ double geo[GQ * GS];
@@ -735,7 +741,7 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
     if (wave == 0) {
         fill_tables3(es.nq, lane, tv, td);
     } else {
-        for (int r = tid - 64; r < BR; r += WG - 64)
+        for (int r = tid - 64; r < R; r += WG - 64)
             dsp[r] = r < n ? (SPACE == 0 ? desc_h1(r, es.o) : desc_hcurl(r, es.o)) : 0u;
         for (int k = tid - 64; k < a.ngeo; k += WG - 64) dgeo[k] = desc_h1(k, es.o);
     }
@@ -752,16 +758,45 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
     __syncthreads();
 
     const double coef = a.coef[e];
-    // this thread's row of A and its half of each chunk's points
-    const int row = tid & (BR - 1), half = tid >> 7;
+    // this thread's row of A and its share sub of each chunk's points
+    const int row = tid & (R - 1), sub = tid / R;
     const unsigned drow = dsp[row];
     const bool row_ok = row < n;
-    // slots: tile row w (tiles w..7) then tile row 7-w (tiles 7-w..7): 9 per wave
-    const int tiA = wave, tiB = 7 - wave, nA = 8 - wave;
+    // tile slots: s < nA in tile row tiA (tile columns tjA + s), the rest in
+    // tile row tiB (tile columns tjB + s - nA).  128 rows: wave w holds tile
+    // rows w and 7 - w (9 tiles).  64 rows: waves p and p + 2 share tile rows
+    // p and 3 - p (5 tiles: 3 + 2).
+    int tiA, tjA, nA, tiB, tjB, nslot;
+    if constexpr (R == BR) {
+        tiA = tjA = wave;
+        nA = 8 - wave;
+        tiB = tjB = 7 - wave;
+        nslot = 9;
+    } else {
+        const int p = wave & 1;
+        tiA = p;
+        tiB = tjB = 3 - p;
+        if (wave < 2) {
+            tjA = p;
+            nA = nslot = 3;
+        } else {
+            tjA = p + 3;
+            nA = 1 - p;
+            nslot = 2;
+        }
+    }
+    auto slot_tile = [&](int s, int &ti, int &tj) {
+        ti = s < nA ? tiA : tiB;
+        tj = s < nA ? tjA + s : tjB + (s - nA);
+    };
     const int offA = tiA * 16 + (lane & 15), offB = tiB * 16 + (lane & 15);
     int offS[NSLOT];
 #pragma unroll
-    for (int s = 0; s < NSLOT; ++s) offS[s] = (s < nA ? tiA + s : s - 1) * 16 + (lane & 15);
+    for (int s = 0; s < NSLOT; ++s) {
+        int ti, tj;
+        slot_tile(s, ti, tj);
+        offS[s] = tj * 16 + (lane & 15);
+    }
     typedef Carry<T, acc_t> CY;
     acc_t acc[NSLOT];
     typename CY::hi_t hi[NSLOT];
@@ -773,15 +808,16 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
     unsigned valid = 0;
 #pragma unroll
     for (int s = 0; s < NSLOT; ++s) {
-        int ti = s < nA ? tiA : tiB, tj = s < nA ? tiA + s : s - 1;
-        valid |= ((ti * 16 < n && tj * 16 < n) ? 1u : 0u) << s;
+        int ti, tj;
+        slot_tile(s, ti, tj);
+        valid |= ((s < nslot && ti * 16 < n && tj * 16 < n) ? 1u : 0u) << s;
     }
     double bsum = 0.0;
 
     auto build = [&](int g0, int l0, int gend, T *buf) {
 #pragma unroll
-        for (int qq = 0; qq < QC / 2; ++qq) {
-            const int q = 2 * qq + half, lq = l0 + q;
+        for (int qq = 0; qq < QC / SH; ++qq) {
+            const int q = SH * qq + sub, lq = l0 + q;
             double out[NC];
 #pragma unroll
             for (int c = 0; c < NC; ++c) out[c] = 0.0;
@@ -791,7 +827,7 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
                 bsum += a_entries_load<SPACE>(drow, p, tv, td, geo + lq * GS, out);
             }
 #pragma unroll
-            for (int c = 0; c < NC; ++c) buf[(NC * q + c) * LDS1 + row] = (T)out[c];
+            for (int c = 0; c < NC; ++c) buf[(NC * q + c) * LD1 + row] = (T)out[c];
         }
     };
     // The slot mask is wave-uniform.  When some of this wave's tiles lie beyond
@@ -803,7 +839,7 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
         if (all_live) {
 #pragma unroll
             for (int k0 = 0; k0 < KCS; k0 += 4) {
-                const T *col = buf + (k0 + (lane >> 4)) * LDS1;
+                const T *col = buf + (k0 + (lane >> 4)) * LD1;
                 const T aA = col[offA], aB = col[offB];
 #pragma unroll
                 for (int s = 0; s < NSLOT; ++s) acc[s] = M::mma(s < nA ? aA : aB, col[offS[s]], acc[s]);
@@ -811,7 +847,7 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
         } else {
 #pragma unroll
             for (int k0 = 0; k0 < KCS; k0 += 4) {
-                const T *col = buf + (k0 + (lane >> 4)) * LDS1;
+                const T *col = buf + (k0 + (lane >> 4)) * LD1;
                 const T aA = col[offA], aB = col[offB];
 #pragma unroll
                 for (int s = 0; s < NSLOT; ++s)
@@ -850,21 +886,24 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
     // ---- outputs: load vector, K, and the condensed system ----
     // (the main loop ended on a barrier: the A buffers are free)
     double *kb = smem;              // bubble rows b < nb: Abi cols 0..ni-1, Abb cols ni..n-1, Bb col n
-    double *wm = kb + NBF * LDS1;   // L_u^-1, leading dimension 33
-    double *rd = wm + NBF * 33;     // 1 / D(j)
-    double *bv = rd + NBF;          // load vector
-    double *stg = bv + LDS1;        // 4 x 16 x 17 wave stages
+    double *wm = kb + NBX * LD1;    // L_u^-1, leading dimension LW
+    double *rd = wm + NBX * LW;     // 1 / D(j)
+    double *bv = rd + NBX;          // load vector
+    double *stg = bv + LD1;         // 4 x 16 x 17 wave stages
+    double *bp = stg + 4 * 16 * 17; // partial load sums of shares 1..SH-1
     T *kstage = reinterpret_cast<T *>(stg + wave * (16 * 17));
-    if (half == 1) bv[row] = bsum;
+    if (sub > 0) bp[(sub - 1) * R + row] = bsum;
     __syncthreads();
-    if (half == 0) bsum += bv[row];
+    if (sub == 0)
+#pragma unroll
+        for (int k = 0; k < SH - 1; ++k) bsum += bp[k * R + row];
     const int nb = es.nb, ni = n - nb;
-    const bool cond = cd.aii != nullptr && nb <= NBF;
+    const bool cond = cd.aii != nullptr && nb <= NBX;
     __syncthreads();
-    if (half == 0 && row_ok) {
+    if (sub == 0 && row_ok) {
         if (a.bout) a.bout[a.off_v[e] + row] = bsum;
         bv[row] = bsum;
-        if (cond && row >= ni) kb[(row - ni) * LDS1 + n] = bsum;
+        if (cond && row >= ni) kb[(row - ni) * LD1 + n] = bsum;
     }
     auto write_k = [&]() {
         T *kout = (T *)a.kout + a.off_k[e];
@@ -872,28 +911,32 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
 #pragma unroll
         for (int s = 0; s < NSLOT; ++s) {
             if (!(valid & (1u << s))) continue;
-            int ti = s < nA ? tiA : tiB, tj = s < nA ? tiA + s : s - 1;
+            int ti, tj;
+            slot_tile(s, ti, tj);
             write_sym_tile<acc_t, T, M>(CY::sum(hi[s], acc[s]), ti * 16, tj * 16, n, kout, ld, kstage, lane);
         }
     };
     if (!cond) {
         if (a.kout) write_k();
-        if (tid == 0 && a.status) a.status[e] = es.negjac ? HP3D_NEG_JACOBIAN : 0;
+        // an element beyond the bubble limit the caller declared is an error
+        if (tid == 0 && a.status)
+            a.status[e] = cd.aii != nullptr ? HP3D_E_ARG : (es.negjac ? HP3D_NEG_JACOBIAN : 0);
         return;
     }
     // bubble rows of K from the register tiles; L_u^-1 starts as the identity
-    for (int idx = tid; idx < NBF * 33; idx += WG) wm[idx] = (idx / 33 == idx % 33) ? 1.0 : 0.0;
+    for (int idx = tid; idx < NBX * LW; idx += WG) wm[idx] = (idx / LW == idx % LW) ? 1.0 : 0.0;
 #pragma unroll
     for (int s = 0; s < NSLOT; ++s) {
         if (!(valid & (1u << s))) continue;
-        int ti = s < nA ? tiA : tiB, tj = s < nA ? tiA + s : s - 1;
+        int ti, tj;
+        slot_tile(s, ti, tj);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             int r = ti * 16 + M::row(lane, j), c = tj * 16 + (lane & 15);
             double v = CY::val(hi[s], acc[s], j);
             if (r < n && c < n) {
-                if (c >= ni) kb[(c - ni) * LDS1 + r] = v;
-                if (r >= ni) kb[(r - ni) * LDS1 + c] = v;
+                if (c >= ni) kb[(c - ni) * LD1 + r] = v;
+                if (r >= ni) kb[(r - ni) * LD1 + c] = v;
             }
         }
     }
@@ -902,22 +945,22 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
     // Gaussian elimination without pivoting of [Abb | I] (Abb = L_u D L_u^T,
     // SPD: every pivot D(j) > 0): the Abb block becomes U = D L_u^T and the
     // identity block L_u^-1.  Items (row i, column c) of a step are spread
-    // over the workgroup, 32 columns per row.
-    double *li = wm; // L_u^-1, leading dimension 33 (unit lower; zero above)
+    // over the workgroup, NBX columns per row.
+    double *li = wm; // L_u^-1, leading dimension LW (unit lower; zero above)
     for (int j = 0; j < nb; ++j) {
-        const double piv = kb[j * LDS1 + ni + j];
+        const double piv = kb[j * LD1 + ni + j];
         const double rp = piv > 0.0 ? 1.0 / piv : 0.0;
         if (tid == 0) {
             rd[j] = rp;
             if (!(piv > 0.0)) s_fail = 1;
         }
-        const int nit = (nb - j - 1) * NBF;
+        const int nit = (nb - j - 1) * NBX;
         for (int it = tid; it < nit; it += WG) {
-            const int i = j + 1 + (it / NBF), c = it % NBF;
+            const int i = j + 1 + (it / NBX), c = it % NBX;
             if (c >= nb) continue;
-            const double m = kb[i * LDS1 + ni + j] * rp;
-            if (c > j) kb[i * LDS1 + ni + c] -= m * kb[j * LDS1 + ni + c];
-            else li[i * 33 + c] -= m * li[j * 33 + c];
+            const double m = kb[i * LD1 + ni + j] * rp;
+            if (c > j) kb[i * LD1 + ni + c] -= m * kb[j * LD1 + ni + c];
+            else li[i * LW + c] -= m * li[j * LW + c];
         }
         __syncthreads();
     }
@@ -929,7 +972,7 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
     auto rhs_col = [&](int c) { return c < ni ? c : (c == ni ? n : -1); };
     auto gemm_li = [&](bool second, d4_t *out) {
 #pragma unroll
-        for (int u = 0; u < 5; ++u) {
+        for (int u = 0; u < UMAX; ++u) {
             int t = wave + 4 * u;
             out[u] = d4_t{0, 0, 0, 0};
             if (t >= ntile) continue;
@@ -938,15 +981,15 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
             for (int k0 = 0; k0 < nb; k0 += 4) {
                 int k = k0 + (lane >> 4);
                 double av = 0.0, bw = 0.0;
-                if (ar < nb && k < nb) av = second ? li[k * 33 + ar] : li[ar * 33 + k];
-                if (k < nb && pc >= 0) bw = second ? kb[k * LDS1 + pc] * rd[k] : kb[k * LDS1 + pc];
+                if (ar < nb && k < nb) av = second ? li[k * LW + ar] : li[ar * LW + k];
+                if (k < nb && pc >= 0) bw = second ? kb[k * LD1 + pc] * rd[k] : kb[k * LD1 + pc];
                 out[u] = MD::mma(av, bw, out[u]);
             }
         }
     };
     auto store_rhs = [&](const d4_t *in) {
 #pragma unroll
-        for (int u = 0; u < 5; ++u) {
+        for (int u = 0; u < UMAX; ++u) {
             int t = wave + 4 * u;
             if (t >= ntile) continue;
             int tr = t / nct, tc = t % nct;
@@ -954,11 +997,11 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 int b = tr * 16 + MD::row(lane, j);
-                if (b < nb && pc >= 0) kb[b * LDS1 + pc] = in[u][j];
+                if (b < nb && pc >= 0) kb[b * LD1 + pc] = in[u][j];
             }
         }
     };
-    d4_t yt[5];
+    d4_t yt[UMAX];
     if (nb > 0) gemm_li(false, yt);
     __syncthreads();
     if (nb > 0) store_rhs(yt);
@@ -968,28 +1011,29 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
     double *bi = cd.bi + a.off_v[e];
     if (tid < ni) {
         double sacc = bv[tid];
-        for (int k = 0; k < nb; ++k) sacc -= kb[k * LDS1 + tid] * rd[k] * kb[k * LDS1 + n];
+        for (int k = 0; k < nb; ++k) sacc -= kb[k * LD1 + tid] * rd[k] * kb[k * LD1 + n];
         bi[tid] = sacc;
     }
     // Aii = Kii - Z^T D^-1 Z on the register tiles, in FP64
 #pragma unroll
     for (int s = 0; s < NSLOT; ++s) {
         if (!(valid & (1u << s))) continue;
-        int ti = s < nA ? tiA : tiB, tj = s < nA ? tiA + s : s - 1;
-        int R = ti * 16, C = tj * 16;
-        if (R >= ni || C >= ni) continue;
+        int ti, tj;
+        slot_tile(s, ti, tj);
+        const int R0 = ti * 16, C0 = tj * 16;
+        if (R0 >= ni || C0 >= ni) continue;
         d4_t da; // the tile in the accumulator's layout, promoted to FP64
 #pragma unroll
         for (int j = 0; j < 4; ++j) da[j] = CY::val(hi[s], acc[s], j);
         d4_t up = d4_t{0, 0, 0, 0};
         for (int k0 = 0; k0 < nb; k0 += 4) {
             int k = k0 + (lane >> 4);
-            int ra = R + (lane & 15), cb = C + (lane & 15);
-            double av = (k < nb && ra < ni) ? kb[k * LDS1 + ra] * rd[k] : 0.0;
-            double bw = (k < nb && cb < ni) ? kb[k * LDS1 + cb] : 0.0;
+            int ra = R0 + (lane & 15), cb = C0 + (lane & 15);
+            double av = (k < nb && ra < ni) ? kb[k * LD1 + ra] * rd[k] : 0.0;
+            double bw = (k < nb && cb < ni) ? kb[k * LD1 + cb] : 0.0;
             up = MD::mma(av, bw, up);
         }
-        // up[j] is (Z^T D^-1 Z)(R + MD::row(lane,j), C + lane%16): bring it to
+        // up[j] is (Z^T D^-1 Z)(R0 + MD::row(lane,j), C0 + lane%16): bring it to
         // the accumulator's row order through this wave's stage
         double *st = stg + wave * (16 * 17);
 #pragma unroll
@@ -998,7 +1042,7 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
 #pragma unroll
         for (int j = 0; j < 4; ++j) da[j] -= st[M::row(lane, j) * 17 + (lane & 15)];
         wave_sync_lds();
-        write_sym_tile<d4_t, double, M>(da, R, C, ni, aii, (size_t)ni, st, lane);
+        write_sym_tile<d4_t, double, M>(da, R0, C0, ni, aii, (size_t)ni, st, lane);
     }
     // X = L_u^-T D^-1 Z = Abb^-1 [Abi | Bb]
     if (nb > 0) gemm_li(true, yt);
@@ -1007,26 +1051,28 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
     __syncthreads();
     // ASchur (nb x ni, leading dimension nb) and BSchur, coalesced
     double *xs = cd.aschur + cd.off_s[e];
-    for (int idx = tid; idx < nb * ni; idx += WG) xs[idx] = kb[(idx % nb) * LDS1 + idx / nb];
-    if (tid < nb) cd.bschur[a.off_v[e] + tid] = kb[tid * LDS1 + n];
+    for (int idx = tid; idx < nb * ni; idx += WG) xs[idx] = kb[(idx % nb) * LD1 + idx / nb];
+    if (tid < nb) cd.bschur[a.off_v[e] + tid] = kb[tid * LD1 + n];
     if (tid == 0 && a.status) a.status[e] = s_fail ? HP3D_E_FACTOR : (es.negjac ? HP3D_NEG_JACOBIAN : 0);
 }
 
-// one workgroup per element
-template <typename T, int SPACE>
-__global__ __launch_bounds__(WG, 2) void elem_single_kernel(ElemArgs a, CondArgs cd)
+// one workgroup per element; R = 64 for elements of at most 64 dofs and 16
+// bubbles: half the LDS and a third of the tile registers, so three
+// workgroups share a CU
+template <typename T, int SPACE, int R>
+__global__ __launch_bounds__(WG, R == BR ? 2 : 3) void elem_single_kernel(ElemArgs a, CondArgs cd)
 {
-    single_element<T, SPACE>(a, cd, blockIdx.x);
+    single_element<T, SPACE, R>(a, cd, blockIdx.x);
 }
 
 // persistent form: a fixed grid (2 workgroups per CU) walks the elements with
 // a grid stride; the barrier after each element keeps the next element's
 // setup from overwriting LDS a slower wave still reads
-template <typename T, int SPACE>
-__global__ __launch_bounds__(WG, 2) void elem_single_persistent(ElemArgs a, CondArgs cd)
+template <typename T, int SPACE, int R>
+__global__ __launch_bounds__(WG, R == BR ? 2 : 3) void elem_single_persistent(ElemArgs a, CondArgs cd)
 {
     for (int e = blockIdx.x; e < a.nel; e += gridDim.x) {
-        single_element<T, SPACE>(a, cd, e);
+        single_element<T, SPACE, R>(a, cd, e);
         __syncthreads();
     }
 }
@@ -1773,34 +1819,45 @@ static int nblocks_for(int max_ndof)
     return t * (t + 1) / 2;
 }
 
-static int launch_single(int space, int precision, const ElemArgs &args, const CondArgs &cd,
-                         hipStream_t st)
+template <int R>
+static int launch_single_r(int space, int precision, const ElemArgs &args, const CondArgs &cd,
+                           hipStream_t st)
 {
     dim3 block(WG);
     if (g_persistent) {
         int dev = 0, ncu = 0;
         HIPCHK(hipGetDevice(&dev));
         HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-        dim3 grid(std::min(args.nel, 2 * std::max(ncu, 1)));
+        dim3 grid(std::min(args.nel, (R == BR ? 2 : 3) * std::max(ncu, 1)));
         if (space == HP3D_H1) {
-            if (precision == HP3D_FP64) hipLaunchKernelGGL((elem_single_persistent<double, 0>), grid, block, 0, st, args, cd);
-            else hipLaunchKernelGGL((elem_single_persistent<float, 0>), grid, block, 0, st, args, cd);
+            if (precision == HP3D_FP64) hipLaunchKernelGGL((elem_single_persistent<double, 0, R>), grid, block, 0, st, args, cd);
+            else hipLaunchKernelGGL((elem_single_persistent<float, 0, R>), grid, block, 0, st, args, cd);
         } else {
-            if (precision == HP3D_FP64) hipLaunchKernelGGL((elem_single_persistent<double, 1>), grid, block, 0, st, args, cd);
-            else hipLaunchKernelGGL((elem_single_persistent<float, 1>), grid, block, 0, st, args, cd);
+            if (precision == HP3D_FP64) hipLaunchKernelGGL((elem_single_persistent<double, 1, R>), grid, block, 0, st, args, cd);
+            else hipLaunchKernelGGL((elem_single_persistent<float, 1, R>), grid, block, 0, st, args, cd);
         }
     } else {
         dim3 grid(args.nel);
         if (space == HP3D_H1) {
-            if (precision == HP3D_FP64) hipLaunchKernelGGL((elem_single_kernel<double, 0>), grid, block, 0, st, args, cd);
-            else hipLaunchKernelGGL((elem_single_kernel<float, 0>), grid, block, 0, st, args, cd);
+            if (precision == HP3D_FP64) hipLaunchKernelGGL((elem_single_kernel<double, 0, R>), grid, block, 0, st, args, cd);
+            else hipLaunchKernelGGL((elem_single_kernel<float, 0, R>), grid, block, 0, st, args, cd);
         } else {
-            if (precision == HP3D_FP64) hipLaunchKernelGGL((elem_single_kernel<double, 1>), grid, block, 0, st, args, cd);
-            else hipLaunchKernelGGL((elem_single_kernel<float, 1>), grid, block, 0, st, args, cd);
+            if (precision == HP3D_FP64) hipLaunchKernelGGL((elem_single_kernel<double, 1, R>), grid, block, 0, st, args, cd);
+            else hipLaunchKernelGGL((elem_single_kernel<float, 1, R>), grid, block, 0, st, args, cd);
         }
     }
     HIPCHK(hipGetLastError());
     return 0;
+}
+
+// the single-block kernel for a batch whose elements have at most max_ndof
+// dofs and (when condensing) at most max_nbubble bubbles: 64 rows when they fit
+static int launch_single(int space, int precision, const ElemArgs &args, const CondArgs &cd,
+                         int max_ndof, int max_nbubble, hipStream_t st)
+{
+    if (max_ndof <= BR / 2 && (cd.aii == nullptr || max_nbubble <= NBF / 2))
+        return launch_single_r<BR / 2>(space, precision, args, cd, st);
+    return launch_single_r<BR>(space, precision, args, cd, st);
 }
 
 static int launch_elem(int space, int precision, const ElemArgs &args, int max_ndof,
@@ -1808,7 +1865,7 @@ static int launch_elem(int space, int precision, const ElemArgs &args, int max_n
 {
     if (max_ndof <= BR && precision != HP3D_BF16) {
         CondArgs cd{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-        return launch_single(space, precision, args, cd, st);
+        return launch_single(space, precision, args, cd, max_ndof, 0, st);
     }
     dim3 grid(args.nel, nblocks_for(max_ndof)), block(WGE);
     if (space == HP3D_H1) {
@@ -2040,7 +2097,7 @@ int hp3d_gpu_celem_batch(int space, int precision, int nel, const int *norder, c
     hipStream_t st = (hipStream_t)stream;
     if (max_ndof <= BR && max_nbubble <= NBF && precision != HP3D_BF16) {
         CondArgs cd{off_c, off_s, aii, bi, aschur, bschur};
-        return launch_single(space, precision, a, cd, st);
+        return launch_single(space, precision, a, cd, max_ndof, max_nbubble, st);
     }
     // two passes: element matrices, then condensation from HBM
     if (!kout || !bout) return HP3D_E_ARG;

@@ -168,10 +168,14 @@ class ElementBatch:
     FUSED_MAX_NBUBBLE = 32
     # bin rules: an element goes to the first rule whose limits it meets; a
     # "fused" bin runs the single-block kernel (K in registers, condensation in
-    # the same launch; needs N <= 128, nb <= 32), as a persistent grid when the
-    # rule says "persistent": True; a "two_pass" bin the block-tiled element
-    # kernel and then the blocked condensation
-    DEFAULT_RULES = ({"max_ndof": 128, "max_nbubble": 32, "path": "fused"}, {"path": "two_pass"})
+    # the same launch; needs N <= 128, nb <= 32; a bin with N <= 64 and
+    # nb <= 16 gets its 64-row form), as a persistent grid when the rule says
+    # "persistent": True; a "two_pass" bin the block-tiled element kernel and
+    # then the blocked condensation
+    SMALL_MAX_NDOF = 64
+    SMALL_MAX_NBUBBLE = 16
+    DEFAULT_RULES = ({"max_ndof": 64, "max_nbubble": 16, "path": "fused"},
+                     {"max_ndof": 128, "max_nbubble": 32, "path": "fused"}, {"path": "two_pass"})
 
     def __init__(self, space, norder, nori_e, nori_f, xnod, coef, load=LOAD_SINSINSIN,
                  device=None, binned=True, rules=None):

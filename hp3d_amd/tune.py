@@ -5,7 +5,9 @@ tiled on the chip and the limits that route elements between them:
 
 * the single-block kernel: one workgroup holds the element's whole K
   (up to 128 x 128) in MFMA accumulator registers, 9 tiles of 16 x 16 per
-  wave, and condenses the bubbles in the same launch;
+  wave, and condenses the bubbles in the same launch; a bin of elements
+  with N <= 64 and nb <= 16 runs its 64-row form (3 tiles per wave, three
+  workgroups per CU);
 * the block-tiled path: one workgroup per 128 x 128 block of K (or the
   single-block kernel without condensation for small N), K written to HBM,
   then the blocked MFMA condensation;
@@ -14,7 +16,8 @@ tiled on the chip and the limits that route elements between them:
 
 For a mesh, ``autotune`` times (HIP events on the launch stream) the full
 element + condensation step of the batch under each candidate rule set --
-fused-bin limits N <= {64, 96, 128} x nb <= {8, 16, 32}, and everything
+fused-bin limits N <= {64, 96, 128} x nb <= {8, 16, 32}, each with and
+without a 64-row bin (N <= 64, nb <= 16) ahead of it, and everything
 two-pass -- then the persistent form of the winner's fused bins, and
 returns the fastest rules with the table of timings.
 ``ElementBatch(..., rules=best)`` then runs with them.  The OpenMP loop of
@@ -30,11 +33,15 @@ from .elem import ElementBatch, ndof
 
 
 def candidate_rules():
+    small = {"max_ndof": 64, "max_nbubble": 16, "path": "fused"}
     out = []
     for nmax in (64, 96, 128):
         for nbmax in (8, 16, 32):
-            out.append(({"max_ndof": nmax, "max_nbubble": nbmax, "path": "fused"},
-                        {"path": "two_pass"}))
+            fused = {"max_ndof": nmax, "max_nbubble": nbmax, "path": "fused"}
+            out.append((fused, {"path": "two_pass"}))
+            if nmax > 64 or nbmax > 16:
+                out.append((small, fused, {"path": "two_pass"}))
+    out.append((small, {"path": "two_pass"}))
     out.append(({"path": "two_pass"},))
     return out
 

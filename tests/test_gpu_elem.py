@@ -149,8 +149,8 @@ def test_dropin_elem_and_stc(gpu):
 
 
 def test_order_bins_mixed_mesh_and_graph_replay(gpu):
-    """Mixed anisotropic orders with the minimum rule: both order bins (fused
-    and two-pass) against the oracle, then the captured graph replayed three
+    """Mixed anisotropic orders with the minimum rule: the three order bins
+    (64-row fused, 128-row fused, two-pass) against the oracle, then the captured graph replayed three
     times with eager work in between, each replay equal to the eager result."""
     import torch
 
@@ -162,7 +162,8 @@ def test_order_bins_mixed_mesh_and_graph_replay(gpu):
     m = mesh.structured((5, 4, 3), porder=porder, perturb=0.1, mesh_seed=8, perm_seed=9,
                         coef=np.exp(rng.normal(size=60)))
     batch = hp3d_amd.ElementBatch.from_mesh(0, m)
-    assert len(batch.bins) == 2
+    assert batch.bin_paths == ["fused", "fused", "two_pass"]
+    assert batch.bins[0][2] <= 64 and batch.bins[1][2] > 64
     # bi / bschur use length-N slots of which ni / nb entries are written:
     # start from zeros so the untouched tails compare equal across replays
     out = {k: v.zero_() for k, v in batch.alloc_outputs(0).items()}
@@ -258,7 +259,8 @@ def test_hcurl_fp32_carry(gpu):
 
 @pytest.mark.parametrize("precision", [0, 1])
 def test_persistent_single_block_kernel(gpu, precision):
-    """The persistent grid gives the same bits as one workgroup per element."""
+    """The persistent grid gives the same bits as one workgroup per element
+    (64-row and 128-row bins)."""
     import torch
 
     import hp3d_amd
@@ -267,11 +269,12 @@ def test_persistent_single_block_kernel(gpu, precision):
     porder = rng.choice(np.arange(1, 5), size=(6 * 5 * 4, 3), p=[0.1, 0.4, 0.3, 0.2])
     m = mesh.structured((6, 5, 4), porder=porder, perturb=0.1, mesh_seed=5, perm_seed=6,
                         coef=np.exp(rng.normal(size=120)))
-    plain = hp3d_amd.ElementBatch.from_mesh(0, m)
-    pers = hp3d_amd.ElementBatch.from_mesh(0, m, rules=(
-        {"max_ndof": 128, "max_nbubble": 32, "path": "fused", "persistent": True},
-        {"path": "two_pass"}))
-    assert pers.bin_persistent[0]
+    rules = ({"max_ndof": 64, "max_nbubble": 16, "path": "fused"},
+             {"max_ndof": 128, "max_nbubble": 32, "path": "fused"}, {"path": "two_pass"})
+    plain = hp3d_amd.ElementBatch.from_mesh(0, m, rules=rules)
+    pers = hp3d_amd.ElementBatch.from_mesh(0, m, rules=tuple(
+        dict(r, persistent=True) if r["path"] == "fused" else r for r in rules))
+    assert pers.bin_persistent[:2] == [True, True] and plain.bins == pers.bins
     o1 = plain.run(precision)
     o2 = pers.run(precision)
     torch.cuda.synchronize()
@@ -279,3 +282,40 @@ def test_persistent_single_block_kernel(gpu, precision):
     pers.check_status()
     for key in ("k", "b", "aii", "aschur"):
         assert torch.equal(o1[key], o2[key]), key
+
+
+@pytest.mark.parametrize("space,pmax", [(0, 3), (1, 2)])
+def test_small_fused_kernel(gpu, space, pmax):
+    """The 64-row single-block kernel (N <= 64, nb <= 16) against the oracle in
+    FP64 and against the 128-row kernel in FP64 and FP32."""
+    import torch
+
+    import hp3d_amd
+
+    rng = np.random.default_rng(31 + space)
+    porder = rng.integers(1, pmax + 1, size=(4 * 4 * 3, 3))
+    m = mesh.structured((4, 4, 3), porder=porder, perturb=0.1, mesh_seed=4, perm_seed=5,
+                        coef=np.exp(rng.normal(size=48)))
+    small = hp3d_amd.ElementBatch.from_mesh(space, m)
+    assert small.bin_paths == ["fused"] and small.bins[0][2] <= 64 and small.bins[0][3] <= 16
+    big = hp3d_amd.ElementBatch.from_mesh(space, m, rules=(
+        {"max_ndof": 128, "max_nbubble": 32, "path": "fused"}, {"path": "two_pass"}))
+    ref = oracle.element_loop(space, m.norder, m.nori_e, m.nori_f, m.xnod(), m.coef,
+                              oracle.LOAD_SINSINSIN if space == 0 else oracle.LOAD_CURL_SIN,
+                              ldk=int(small.n.max()), herm=True)
+    for prec, tol in ((0, 1e-13), (1, 1e-6)):
+        o1, o2 = small.run(prec), big.run(prec)
+        torch.cuda.synchronize()
+        small.check_status()
+        big.check_status()
+        if prec == 0:
+            compare(small, o1, ref, TOL64, 1e-11)
+        for e in range(m.nel):
+            ni = int(small.ni[e])
+            assert rel(small.k_of(o1, e), big.k_of(o2, e)) < tol
+            assert rel(small.vec_of(o1, "b", e), big.vec_of(o2, "b", e)) < 1e-13
+            if int(small.nb[e]) == 0:
+                continue
+            assert rel(small.aii_of(o1, e), big.aii_of(o2, e)) < tol
+            assert rel(small.vec_of(o1, "bi", e, ni), big.vec_of(o2, "bi", e, ni)) < tol
+            assert rel(small.aschur_of(o1, e), big.aschur_of(o2, e)) < (tol if prec == 0 else 1e-3)
