@@ -201,7 +201,9 @@ class ElementBatch:
         if (which < 0).any():
             raise ValueError("bin rules leave elements unassigned (the last rule should be open)")
         self.rules = rules
-        self.perm = np.argsort(which, kind="stable")
+        # bin by bin; within a bin the largest elements first, so the last
+        # workgroups of a launch are the cheapest ones
+        self.perm = np.lexsort((-nb, -n, which))
         self.iperm = np.empty_like(self.perm)
         self.iperm[self.perm] = np.arange(self.nel)
         P = self.perm
