@@ -37,7 +37,9 @@ SIGNATURES = {
     "hp3d_gpu_stc_fwd": (_i, [_i, _i, _p, _i, _p, _p, _p]),
     "hp3d_gpu_celem": (_i, [_i, _p, _p, _p, _p, _i, _d, _i, _p, _i, _p, _p, _p]),
     "hp3d_gpu_elem_batch": (_i, [_i, _i, _i, _p, _p, _p, _p, _i, _p, _i, _p, _i, _p, _p, _p,
-                                 _p, _i, _p]),
+                                 _p, _p, _i, _p]),
+    "hp3d_gpu_elem_workspace": (_sz, [_i, _i, _i, _i]),
+    "hp3d_gpu_celem_workspace": (_sz, [_i, _i, _i, _i, _i]),
     "hp3d_gpu_stc_batch": (_i, [_i, _i, _i, _p, _p, _i, _p, _p, _p, _p, _i, _p, _p, _p, _p,
                                 _p, _p, _p, _i, _i, _p]),
     "hp3d_gpu_stc_workspace": (_sz, [_i, _i, _i]),

@@ -159,7 +159,14 @@ struct ElemArgs {
     void *kout;
     double *bout;
     int *status;
+    double *gws; // per-element geometry slabs of the block-tiled path
 };
+
+// geometry doubles per quadrature point, and the per-element slab of the
+// block-tiled path: MAXQ points, then the negative-Jacobian flag
+__host__ __device__ constexpr int geo_stride(int space) { return space == 0 ? 10 : 21; }
+__host__ __device__ constexpr size_t geo_slab(int space) { return (size_t)MAXQ * geo_stride(space) + 2; }
+constexpr int XS_MAX = 27; // geometry nodes staged in LDS by the geometry code
 
 __device__ inline void getf(int load, const double *x, double *f)
 {
@@ -456,11 +463,44 @@ __device__ inline s4_t bf4(const float *col, int ld, int off)
     return s4_t{f2bf(col[off]), f2bf(col[ld + off]), f2bf(col[2 * ld + off]), f2bf(col[3 * ld + off])};
 }
 
+// Geometry pass of the block-tiled path: every quadrature point of an
+// element once (geom3D.F90:77-91, geom.F90:57-113, elem_opt.F90:101-122),
+// into the element's slab of the workspace, which all of the element's K
+// blocks then read; one workgroup per element.
+template <int SPACE>
+__global__ __launch_bounds__(WG) void geometry_kernel(ElemArgs a)
+{
+    constexpr int GS = geo_stride(SPACE);
+    __shared__ ElemShared es;
+    __shared__ double tv[3][NTAB * NPT], td[3][NTAB * NPT];
+    __shared__ unsigned dgeo[MAXN_H1];
+    __shared__ double xg[3 * XS_MAX];
+    const int e = blockIdx.x, tid = threadIdx.x, wave = tid >> 6;
+    if (!element_setup<SPACE>(a, e, es)) return; // the element kernel reports the orders
+    if (wave < 3) fill_tables(es.nq[wave], tv[wave], td[wave]);
+    for (int k = tid; k < a.ngeo; k += WG) dgeo[k] = desc_h1(k, es.o);
+    const double *xn = a.xnod + 3 * (size_t)a.ngeo * e;
+    const bool xs_ok = a.ngeo <= XS_MAX;
+    if (xs_ok)
+        for (int k = tid; k < 3 * a.ngeo; k += WG) xg[k] = xn[k];
+    __syncthreads();
+    double *gp = a.gws + (size_t)e * geo_slab(SPACE);
+    const double coef = a.coef[e];
+    bool neg = false;
+    for (int l = tid; l < es.nint; l += WG)
+        neg |= xs_ok ? point_geometry<SPACE>(es, l, tv, td, dgeo, xg, a.ngeo, coef, a.load, gp + l * GS)
+                     : point_geometry<SPACE>(es, l, tv, td, dgeo, xn, a.ngeo, coef, a.load, gp + l * GS);
+    if (neg) es.negjac = 1;
+    __syncthreads();
+    if (tid == 0) gp[(size_t)MAXQ * GS] = es.negjac ? 1.0 : 0.0;
+}
+
 // General element kernel: one workgroup of 8 waves per (element, 128 x 128
 // block of K).  An off-diagonal block's 64 tiles of 16 x 16 are 8 per wave
 // (wave w: tile row w); a diagonal block's 36 upper tiles are 5 or 4 per wave
 // (waves p and p + 4 share tile rows p and 7 - p, 9 tiles).  With 8 tiles a
 // wave fits in 128 VGPRs, so two workgroups give 4 waves per SIMD.
+// The geometry comes from the element's slab (geometry_kernel).
 // Each row of the block's A is built by 2 (off-diagonal) or 4 (diagonal)
 // threads, each for its share of the chunk's points, with the row's
 // descriptor read once; the A chunks are double buffered (FP32 24, FP64 12
@@ -480,7 +520,7 @@ __global__ __launch_bounds__(WGE, 4) void elem_kernel(ElemArgs a)
     constexpr int NBUF = BF ? 1 : 2;                   // A buffers
     constexpr int QC = KCE / NC;                       // points per A chunk
     constexpr int GQ = 64;                             // points per geometry pass
-    constexpr int GS = SPACE == 0 ? 10 : 21;           // geometry doubles per point
+    constexpr int GS = geo_stride(SPACE);              // geometry doubles per point
     constexpr int KCA = BF ? 32 : KCE;                 // allocated A columns
     constexpr int BUFE = KCA * LDA;                    // one A buffer, in T
     constexpr int NS = 8;                              // tile slots per wave
@@ -490,7 +530,6 @@ __global__ __launch_bounds__(WGE, 4) void elem_kernel(ElemArgs a)
     __shared__ ElemShared es;
     __shared__ double tv[3][NTAB * NPT], td[3][NTAB * NPT];
     __shared__ unsigned dsp[2 * BR];   // descriptors of the block's rows
-    __shared__ unsigned dgeo[MAXN_H1]; // H1 descriptors of the geometry dofs
     __shared__ unsigned pix[MAXQ];     // packed 1D point indices
     __shared__ double geo[GQ * GS];
     __shared__ double bv[3][BR];       // partial load sums of a diagonal block's rows
@@ -514,7 +553,6 @@ __global__ __launch_bounds__(WGE, 4) void elem_kernel(ElemArgs a)
         int g = (r < BR ? I * BR + r : J * BR + r - BR);
         dsp[r] = g < n ? (SPACE == 0 ? desc_h1(g, es.o) : desc_hcurl(g, es.o)) : 0u;
     }
-    for (int k = tid; k < a.ngeo; k += WGE) dgeo[k] = desc_h1(k, es.o);
     for (int l = tid; l < nint; l += WGE) {
         Pt p = point_ix(l, es.nq);
         pix[l] = (unsigned)p.l1 | ((unsigned)p.l2 << 4) | ((unsigned)p.l3 << 8);
@@ -523,8 +561,7 @@ __global__ __launch_bounds__(WGE, 4) void elem_kernel(ElemArgs a)
         for (int i = tid; i < (KCA - KC) * LDA; i += WGE) ac[KC * LDA + i] = (T)0;
     __syncthreads();
 
-    const double *xn = a.xnod + 3 * (size_t)a.ngeo * e;
-    const double coef = a.coef[e];
+    const double *gsrc = a.gws + (size_t)e * geo_slab(SPACE);
 
     // tile slots of this wave: slots s < nA in tile row tiA (tile columns
     // tjA + s), the rest in tile row tiB (tile columns tjB + s - nA)
@@ -629,9 +666,7 @@ __global__ __launch_bounds__(WGE, 4) void elem_kernel(ElemArgs a)
     int nfold = 0;
     for (int g0 = 0; g0 < nint; g0 += GQ) {
         const int gend = min(GQ, nint - g0);
-        if (tid < gend &&
-            point_geometry<SPACE>(es, g0 + tid, tv, td, dgeo, xn, a.ngeo, coef, a.load, geo + tid * GS))
-            es.negjac = 1;
+        for (int i = tid; i < gend * GS; i += WGE) geo[i] = gsrc[(size_t)g0 * GS + i];
         __syncthreads();
         const int nch = (gend + QC - 1) / QC;
         build(g0, 0, gend, ac);
@@ -659,7 +694,8 @@ __global__ __launch_bounds__(WGE, 4) void elem_kernel(ElemArgs a)
     if (do_load && sub > 0) bv[sub - 1][row] = bsum;
     __syncthreads();
     if (do_load && sub == 0 && row_ok) a.bout[a.off_v[e] + grow] = bsum + bv[0][row] + bv[1][row] + bv[2][row];
-    if (tid == 0 && blockIdx.y == 0 && a.status) a.status[e] = es.negjac ? HP3D_NEG_JACOBIAN : 0;
+    if (tid == 0 && blockIdx.y == 0 && a.status)
+        a.status[e] = gsrc[(size_t)MAXQ * GS] != 0.0 ? HP3D_NEG_JACOBIAN : 0;
 
     T *kout = (T *)a.kout + a.off_k[e];
     const size_t ld = a.ld_k > 0 ? (size_t)a.ld_k : (size_t)n;
@@ -686,7 +722,6 @@ __global__ __launch_bounds__(WGE, 4) void elem_kernel(ElemArgs a)
 // the VALU/LDS work that builds chunk c+1, so the two overlap.
 constexpr int NBF = 32;      // largest bubble block condensed in the fused path
 constexpr int LDS1 = BR + 16; // LDS stride of the single-block kernel's A columns (see LDA)
-constexpr int XS_MAX = 27;   // geometry nodes staged in LDS by the single-block kernel
 struct CondArgs {
     const long long *off_c, *off_s;
     double *aii, *bi, *aschur, *bschur;
@@ -1867,6 +1902,9 @@ static int launch_elem(int space, int precision, const ElemArgs &args, int max_n
         CondArgs cd{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
         return launch_single(space, precision, args, cd, max_ndof, 0, st);
     }
+    if (!args.gws) return HP3D_E_ARG; // the block-tiled path needs the geometry workspace
+    if (space == HP3D_H1) hipLaunchKernelGGL((geometry_kernel<0>), dim3(args.nel), dim3(WG), 0, st, args);
+    else hipLaunchKernelGGL((geometry_kernel<1>), dim3(args.nel), dim3(WG), 0, st, args);
     dim3 grid(args.nel, nblocks_for(max_ndof)), block(WGE);
     if (space == HP3D_H1) {
         if (precision == HP3D_FP64) hipLaunchKernelGGL((elem_kernel<double, 0>), grid, block, 0, st, args);
@@ -1944,7 +1982,8 @@ int hp3d_gpu_init(int device, int max_ndof)
     HIPCHK(hipMalloc(&g.d_as, nn * sizeof(double)));
     HIPCHK(hipMalloc(&g.d_bs, max_ndof * sizeof(double)));
     g.nbm = max_bubbles(HP3D_HCURL, max_ndof);
-    HIPCHK(hipMalloc(&g.d_ws, stc_slab(rnd16(g.nbm), rnd16(max_ndof + 1)) * sizeof(double)));
+    HIPCHK(hipMalloc(&g.d_ws, std::max(stc_slab(rnd16(g.nbm), rnd16(max_ndof + 1)), geo_slab(HP3D_HCURL)) *
+                                  sizeof(double)));
     HIPCHK(hipMalloc(&g.d_off, sizeof(long long)));
     HIPCHK(hipMemset(g.d_off, 0, sizeof(long long)));
     HIPCHK(hipMalloc(&g.d_status, 2 * sizeof(int)));
@@ -1977,14 +2016,15 @@ int hp3d_gpu_finalize(void)
 int hp3d_gpu_elem_batch(int space, int precision, int nel, const int *norder, const int *nori_e,
                         const int *nori_f, const double *xnod, int ngeo, const double *coef,
                         int load, const long long *off_k, int ld_k, void *kout,
-                        const long long *off_v, double *bout, int *status, int max_ndof,
-                        void *stream)
+                        const long long *off_v, double *bout, void *workspace, int *status,
+                        int max_ndof, void *stream)
 {
     if ((space != HP3D_H1 && space != HP3D_HCURL) || (precision != HP3D_FP64 && precision != HP3D_FP32 && precision != HP3D_BF16))
         return HP3D_E_ARG;
     if (nel < 0 || ngeo < 1 || ngeo > MAXN_H1 || max_ndof < 1 || ld_k < 0) return HP3D_E_ARG;
     if (nel == 0) return 0;
     if (!norder || !nori_e || !nori_f || !xnod || !coef || !off_k || !kout) return HP3D_E_ARG;
+    if (!workspace && hp3d_gpu_elem_workspace(space, precision, nel, max_ndof) > 0) return HP3D_E_ARG;
     if (bout && !off_v) return HP3D_E_ARG;
     if (ld_k > 0 && ld_k < max_ndof) return HP3D_E_ARG;
     ElemArgs a;
@@ -2004,13 +2044,29 @@ int hp3d_gpu_elem_batch(int space, int precision, int nel, const int *norder, co
     a.kout = kout;
     a.bout = bout;
     a.status = status;
+    a.gws = (double *)workspace;
     return launch_elem(space, precision, a, max_ndof, (hipStream_t)stream);
+}
+
+size_t hp3d_gpu_elem_workspace(int space, int precision, int nel, int max_ndof)
+{
+    if (nel <= 0 || max_ndof <= 0 || (space != HP3D_H1 && space != HP3D_HCURL)) return 0;
+    if (max_ndof <= BR && precision != HP3D_BF16) return 0; // single-block kernel
+    return (size_t)nel * geo_slab(space) * sizeof(double);
 }
 
 size_t hp3d_gpu_stc_workspace(int nel, int max_ndof, int max_nbubble)
 {
     if (nel <= 0 || max_nbubble <= 0 || max_ndof <= 0) return 0;
     return (size_t)nel * stc_slab(rnd16(max_nbubble), rnd16(max_ndof + 1)) * sizeof(double);
+}
+
+size_t hp3d_gpu_celem_workspace(int space, int precision, int nel, int max_ndof, int max_nbubble)
+{
+    if (max_ndof <= BR && max_nbubble <= NBF && precision != HP3D_BF16) return 0; // fused
+    // the geometry pass and the condensation use the same memory one after the other
+    return std::max(hp3d_gpu_elem_workspace(space, precision, nel, max_ndof),
+                    hp3d_gpu_stc_workspace(nel, max_ndof, max_nbubble));
 }
 
 int hp3d_gpu_stc_batch(int space, int precision, int nel, const int *norder,
@@ -2094,6 +2150,7 @@ int hp3d_gpu_celem_batch(int space, int precision, int nel, const int *norder, c
     a.kout = kout;
     a.bout = bout;
     a.status = status;
+    a.gws = (double *)workspace;
     hipStream_t st = (hipStream_t)stream;
     if (max_ndof <= BR && max_nbubble <= NBF && precision != HP3D_BF16) {
         CondArgs cd{off_c, off_s, aii, bi, aschur, bschur};
@@ -2196,7 +2253,7 @@ static int elem_one(int space, const int norder[19], const int nori_e[12], const
     HIPCHK(hipMemcpyAsync(g.d_coef, g.h_x + 3 * ngeo, sizeof(double), hipMemcpyHostToDevice, g.stream));
     int r = hp3d_gpu_elem_batch(space, HP3D_FP64, 1, g.d_codes, g.d_codes + 19, g.d_codes + 31,
                                 g.d_xnod, ngeo, g.d_coef, load, g.d_off, 0, g.d_k, g.d_off, g.d_b,
-                                g.d_status, n, g.stream);
+                                g.d_ws, g.d_status, n, g.stream);
     return r;
 }
 

@@ -251,9 +251,27 @@ class ElementBatch:
         self.off_c = t(self.off_c_h, torch.int64)
         self.off_s = t(self.off_s_h, torch.int64)
         self.status = torch.zeros(self.nel, dtype=torch.int32, device=dev)
+        self.workspace = torch.empty(self._workspace_bytes(FP32) // 8 + 1, dtype=torch.float64, device=dev)
+
+    def _workspace_bytes(self, precision):
+        """Workspace of the per-bin calls: the geometry of the block-tiled
+        element pass and the condensation slabs (one after the other)."""
         lib = _lib.load()
-        ws = max([lib.hp3d_gpu_stc_workspace(cnt, mnd, mnb) for _, cnt, mnd, mnb in self.bins] + [8])
-        self.workspace = torch.empty(max(int(ws) // 8, 1), dtype=torch.float64, device=dev)
+        ws = 8
+        for _, cnt, mnd, mnb in self.bins:
+            ws = max(ws, int(lib.hp3d_gpu_stc_workspace(cnt, mnd, mnb)),
+                     int(lib.hp3d_gpu_elem_workspace(self.space, precision, cnt, mnd)),
+                     int(lib.hp3d_gpu_celem_workspace(self.space, precision, cnt, mnd, mnb)))
+        return ws
+
+    def _ensure_workspace(self, precision):
+        """Grow the workspace for `precision` (the BF16 trial runs every bin
+        through the block-tiled kernel); never called during a capture."""
+        import torch
+
+        need = self._workspace_bytes(precision)
+        if self.workspace.numel() * 8 < need:
+            self.workspace = torch.empty(need // 8 + 1, dtype=torch.float64, device=self.device)
 
     def _ptrs(self, start):
         """Device pointers of the per-element arrays from internal element `start`."""
@@ -294,7 +312,7 @@ class ElementBatch:
             self.space, precision, cnt, at(self.norder, 19), at(self.nori_e, 12),
             at(self.nori_f, 6), at(self.xnod, 3 * self.ngeo), self.ngeo, at(self.coef, 1),
             self.load, at(self.off_k, 1), 0, _tptr(out["k"]), at(self.off_v, 1),
-            _tptr(out["b"]), at(self.status, 1), mnd, ctypes.c_void_p(st))
+            _tptr(out["b"]), _tptr(self.workspace), at(self.status, 1), mnd, ctypes.c_void_p(st))
         check(rc, "hp3d_gpu_elem_batch")
 
     def _condense_bin(self, lib, out, precision, st, lo, cnt, mnd, mnb):
@@ -311,6 +329,7 @@ class ElementBatch:
         """Launch the element kernel(s) into out['k'], out['b'] (no sync)."""
         import torch
 
+        self._ensure_workspace(precision)
         lib = _lib.load()
         st = torch.cuda.current_stream().cuda_stream if stream is None else stream
         for lo, cnt, mnd, _mnb in self.bins:
@@ -331,6 +350,7 @@ class ElementBatch:
         when every bin is fused."""
         import torch
 
+        self._ensure_workspace(precision)
         lib = _lib.load()
         st = torch.cuda.current_stream().cuda_stream if stream is None else stream
         for (lo, cnt, mnd, mnb), path, pers in zip(self.bins, self.bin_paths, self.bin_persistent):
@@ -354,7 +374,8 @@ class ElementBatch:
 
     def capture(self, out, precision=FP64):
         """The per-bin launches of celem() captured in a HIP graph (replay with
-        graph.replay()); inputs and outputs stay where they are."""
+        graph.replay()); inputs and outputs stay where they are (the eager
+        run before the capture sizes the workspace)."""
         import torch
 
         s = torch.cuda.Stream()

@@ -83,12 +83,24 @@ module hp3d_gpu_mod
     end function
 
     integer(c_int) function hp3d_gpu_elem_batch(space, precision, nel, norder, nori_e, nori_f, &
-        xnod, ngeo, coef, load, off_k, ld_k, kout, off_v, bout, status, max_ndof, stream) &
-        bind(C, name="hp3d_gpu_elem_batch")
+        xnod, ngeo, coef, load, off_k, ld_k, kout, off_v, bout, workspace, status, max_ndof, &
+        stream) bind(C, name="hp3d_gpu_elem_batch")
       import :: c_int, c_ptr
       integer(c_int), value :: space, precision, nel, ngeo, load, ld_k, max_ndof
       type(c_ptr), value :: norder, nori_e, nori_f, xnod, coef, off_k, kout, off_v, bout
-      type(c_ptr), value :: status, stream
+      type(c_ptr), value :: workspace, status, stream
+    end function
+
+    integer(c_size_t) function hp3d_gpu_elem_workspace(space, precision, nel, max_ndof) &
+        bind(C, name="hp3d_gpu_elem_workspace")
+      import :: c_int, c_size_t
+      integer(c_int), value :: space, precision, nel, max_ndof
+    end function
+
+    integer(c_size_t) function hp3d_gpu_celem_workspace(space, precision, nel, max_ndof, &
+        max_nbubble) bind(C, name="hp3d_gpu_celem_workspace")
+      import :: c_int, c_size_t
+      integer(c_int), value :: space, precision, nel, max_ndof, max_nbubble
     end function
 
     integer(c_int) function hp3d_gpu_stc_batch(space, precision, nel, norder, off_k, ld_k, kin, &

@@ -133,14 +133,22 @@ int hp3d_gpu_celem(int space, const int norder[19], const int nori_e[12], const 
  *   load at bout + off_v[e] (FP64; bout may be NULL)
  *   status[e] (may be NULL): 0, 1 (negative Jacobian) or HP3D_E_ORDER
  * max_ndof bounds N_e over the batch (it sizes the launch grid).
+ * workspace: device memory of hp3d_gpu_elem_workspace(space, precision, nel,
+ * max_ndof) bytes (the per-element quadrature-point geometry of the
+ * block-tiled kernel, N_e > 128 or BF16); may be NULL when that size is 0.
  * Replaces the element loop of mumps_sc Step 2 up to elem
  * (mumps_sc.F90:253-259 -> celem_systemI.F90:523).
  */
 int hp3d_gpu_elem_batch(int space, int precision, int nel, const int *norder, const int *nori_e,
                         const int *nori_f, const double *xnod, int ngeo, const double *coef,
                         int load, const long long *off_k, int ld_k, void *kout,
-                        const long long *off_v, double *bout, int *status, int max_ndof,
-                        void *stream);
+                        const long long *off_v, double *bout, void *workspace, int *status,
+                        int max_ndof, void *stream);
+/* bytes of device workspace hp3d_gpu_elem_batch needs for nel elements of at
+ * most max_ndof dofs (0 when the single-block kernel serves the batch).  No
+ * counterpart in the reference: its elem computes the geometry point by point
+ * inside the element loop (elem_opt.F90:101-122). */
+size_t hp3d_gpu_elem_workspace(int space, int precision, int nel, int max_ndof);
 
 /*
  * Static condensation of nel element systems already in device memory
@@ -175,9 +183,9 @@ size_t hp3d_gpu_stc_workspace(int nel, int max_ndof, int max_nbubble);
  * element kernel on the register-resident K (no second pass over HBM); kout
  * and bout may then be NULL (K and the full load are not written).  Otherwise
  * K and the load are written and condensed by a second kernel; kout, bout and
- * the hp3d_gpu_stc_workspace(nel, max_ndof, max_nbubble) workspace are then
- * required.  The condensed
- * matrix is packed (leading dimension ni_e).
+ * the hp3d_gpu_celem_workspace(space, precision, nel, max_ndof, max_nbubble)
+ * workspace are then required.  The condensed matrix is packed (leading
+ * dimension ni_e).
  */
 int hp3d_gpu_celem_batch(int space, int precision, int nel, const int *norder, const int *nori_e,
                          const int *nori_f, const double *xnod, int ngeo, const double *coef,
@@ -186,6 +194,11 @@ int hp3d_gpu_celem_batch(int space, int precision, int nel, const int *norder, c
                          double *aii, double *bi, const long long *off_s, double *aschur,
                          double *bschur, void *workspace, int *status, int max_ndof,
                          int max_nbubble, void *stream);
+/* bytes of device workspace hp3d_gpu_celem_batch needs: 0 for the fused path,
+ * else the larger of the element pass's and the condensation's (they use the
+ * same memory one after the other).  No counterpart in the reference (its
+ * THREADPRIVATE workspaces are per thread, mumps_sc.F90:224-342). */
+size_t hp3d_gpu_celem_workspace(int space, int precision, int nel, int max_ndof, int max_nbubble);
 
 /*
  * Back substitution for nel elements (stc_bwd_wrapper + stc_bwd + stc_solout,

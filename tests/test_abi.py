@@ -70,17 +70,25 @@ def test_entries_before_init_and_bad_arguments():
         assert lib.hp3d_gpu_init(0, 0) == _lib.E_INIT
     # argument checks happen before any device work
     assert lib.hp3d_gpu_elem_batch(7, 0, 1, None, None, None, None, 8, None, 0, None, 0, None,
-                                   None, None, None, 1, None) == _lib.E_ARG
+                                   None, None, None, None, 1, None) == _lib.E_ARG
     assert lib.hp3d_gpu_elem_batch(0, 0, 1, None, None, None, None, 8, None, 0, None, 0, None,
-                                   None, None, None, 1, None) == _lib.E_ARG
+                                   None, None, None, None, 1, None) == _lib.E_ARG
     assert lib.hp3d_gpu_stc_batch(0, 3, 1, None, None, 0, None, None, None, None, 0, None, None,
                                   None, None, None, None, None, 8, 1, None) == _lib.E_ARG
     assert lib.hp3d_gpu_elem_batch(0, 0, 0, None, None, None, None, 8, None, 0, None, 0, None,
-                                   None, None, None, 1, None) == 0  # empty batch
+                                   None, None, None, None, 1, None) == 0  # empty batch
     assert lib.hp3d_gpu_stc_workspace(10, 125, 0) == 0
     # slab per element: nb x nb factor + nb x (ni + 1) columns, both rounded up to 16
     assert lib.hp3d_gpu_stc_workspace(10, 125, 27) == 10 * (32 * 32 + 32 * 128) * 8
     assert lib.hp3d_gpu_stc_workspace(10, 540, 240) == 10 * (240 * 240 + 240 * 544) * 8
+    # geometry slabs: one per element of the block-tiled path (343 points x 10 or
+    # 21 doubles + a flag), none for the single-block kernel
+    assert lib.hp3d_gpu_elem_workspace(0, 1, 10, 128) == 0
+    assert lib.hp3d_gpu_elem_workspace(0, 2, 10, 27) == 10 * (343 * 10 + 2) * 8
+    assert lib.hp3d_gpu_elem_workspace(1, 0, 10, 540) == 10 * (343 * 21 + 2) * 8
+    assert lib.hp3d_gpu_celem_workspace(0, 1, 10, 125, 27) == 0
+    assert lib.hp3d_gpu_celem_workspace(1, 1, 10, 540, 240) == max(
+        lib.hp3d_gpu_elem_workspace(1, 1, 10, 540), lib.hp3d_gpu_stc_workspace(10, 540, 240))
 
 
 def test_python_mirror_counts():
