@@ -564,22 +564,25 @@ __global__ __launch_bounds__(WGE, 4) void elem_kernel(ElemArgs a)
     const double *gsrc = a.gws + (size_t)e * geo_slab(SPACE);
 
     // tile slots of this wave: slots s < nA in tile row tiA (tile columns
-    // tjA + s), the rest in tile row tiB (tile columns tjB + s - nA)
+    // tjA + s), the rest in tile row tiB (tile columns tjB + s - nA); all
+    // wave-uniform (scalar registers), with the lane's row within a tile apart
+    const int wv = __builtin_amdgcn_readfirstlane(wave);
+    const int l15 = lane & 15;
     const int nrI = min(BR, n - I * BR), nrJ = min(BR, n - J * BR);
     const int offJ = diag ? 0 : BR;
     int tiA, tiB, tjA, tjB, nA, nslot;
     if (!diag) {
-        tiA = tiB = wave;
+        tiA = tiB = wv;
         tjA = tjB = 0;
         nA = nslot = NS;
-    } else if (wave < 4) {
-        tiA = wave;
-        tiB = 7 - wave;
-        tjA = wave;
+    } else if (wv < 4) {
+        tiA = wv;
+        tiB = 7 - wv;
+        tjA = wv;
         tjB = 0;
         nA = nslot = 5;
     } else {
-        const int p = wave - 4;
+        const int p = wv - 4;
         tiA = p;
         tiB = 7 - p;
         tjA = p + 5;
@@ -587,16 +590,16 @@ __global__ __launch_bounds__(WGE, 4) void elem_kernel(ElemArgs a)
         nA = 3 - p;
         nslot = 4;
     }
-    int offS[NS];
+    int offS[NS]; // column offset of each slot's tile, wave-uniform
     unsigned valid = 0;
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
         const int ti = s < nA ? tiA : tiB, tj = s < nA ? tjA + s : tjB + (s - nA);
-        offS[s] = offJ + tj * 16 + (lane & 15);
+        offS[s] = offJ + tj * 16;
         const bool v = s < nslot && ti * 16 < nrI && tj * 16 < nrJ;
         valid |= (v ? 1u : 0u) << s;
     }
-    const unsigned vmask = __builtin_amdgcn_readfirstlane(valid);
+    const unsigned vmask = valid;
     typedef Carry<std::conditional_t<BF, double, T>, acc_t> CY; // no carry under bfloat16 rounding
     acc_t acc[NS];
     typename CY::hi_t hi[NS];
@@ -610,7 +613,7 @@ __global__ __launch_bounds__(WGE, 4) void elem_kernel(ElemArgs a)
     // this thread's row of A (row r of the block's row set: I rows, then J
     // rows off the diagonal) and its share sub of the chunk's points
     const int nrows = diag ? BR : 2 * BR;
-    const int row = tid & (nrows - 1), sub = tid / nrows;
+    const int row = tid & (nrows - 1), sub = diag ? (wv >> 1) : (wv >> 2);
     const int grow = row < BR ? I * BR + row : J * BR + row - BR;
     const bool row_ok = grow < n;
     const unsigned drow = dsp[row];
@@ -642,9 +645,9 @@ __global__ __launch_bounds__(WGE, 4) void elem_kernel(ElemArgs a)
         if constexpr (BF) {
 #pragma unroll
             for (int k0 = 0; k0 < KCA; k0 += 16) {
-                const float *col = (const float *)buf + (k0 + 4 * (lane >> 4)) * LDA;
-                s4_t aA = bf4(col, LDA, tiA * 16 + (lane & 15));
-                s4_t aB = bf4(col, LDA, tiB * 16 + (lane & 15));
+                const float *col = (const float *)buf + (k0 + 4 * (lane >> 4)) * LDA + l15;
+                s4_t aA = bf4(col, LDA, tiA * 16);
+                s4_t aB = bf4(col, LDA, tiB * 16);
 #pragma unroll
                 for (int s = 0; s < NS; ++s)
                     if (vmask & (1u << s))
@@ -654,8 +657,8 @@ __global__ __launch_bounds__(WGE, 4) void elem_kernel(ElemArgs a)
         } else {
 #pragma unroll
             for (int k0 = 0; k0 < KCE; k0 += 4) {
-                const T *col = buf + (k0 + (lane >> 4)) * LDA;
-                const T aA = col[tiA * 16 + (lane & 15)], aB = col[tiB * 16 + (lane & 15)];
+                const T *col = buf + (k0 + (lane >> 4)) * LDA + l15;
+                const T aA = col[tiA * 16], aB = col[tiB * 16];
 #pragma unroll
                 for (int s = 0; s < NS; ++s)
                     if (vmask & (1u << s)) acc[s] = M::mma(s < nA ? aA : aB, col[offS[s]], acc[s]);
