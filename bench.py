@@ -130,9 +130,16 @@ def cpu_baseline(m, batch, out32, out64, seconds=15.0):
         for tag, o in (("32", out32), ("64", out64)):
             err["k" + tag].append(np.linalg.norm(batch.k_of(o, e) - kr) / np.linalg.norm(kr))
             err["aii" + tag].append(np.linalg.norm(batch.aii_of(o, e) - ar) / np.linalg.norm(ar))
+    # the same loop on one core, on a smaller sample
+    n1 = int(min(ns, max(1, ns / threads / 3)))
+    t0 = time.perf_counter()
+    oracle.element_loop(space, m.norder[:n1], m.nori_e[:n1], m.nori_f[:n1], m.xnod()[:n1],
+                        m.coef[:n1], load, ldk=ldk, nthreads=1)
+    dt1 = time.perf_counter() - t0
     res = {"value": ns / dt, "unit": "element matrices/s", "cores": threads, "kind": "port",
            "sample": "first %d of %d elements, elem_opt + stc_fwd_gen (LU) per element, "
-                     "OpenMP dynamic schedule, %.1f s" % (ns, m.nel, dt)}
+                     "OpenMP dynamic schedule, %.1f s" % (ns, m.nel, dt),
+           "single_core": {"value": n1 / dt1, "sample": "first %d elements, %.1f s" % (n1, dt1)}}
     errors = {k: {"max": float(np.max(v)), "median": float(np.median(v))} for k, v in err.items()}
     return res, errors
 
@@ -282,6 +289,20 @@ def main():
     torch.cuda.synchronize()
     sg = (time.perf_counter() - t0) / args.steps
 
+    # end to end: the FP32 step and the copy of K, the load and the condensed
+    # system into pinned host memory (what a host caller's ALOC/CLOC receive)
+    host = {k: torch.empty(v.shape, dtype=v.dtype, pin_memory=True) for k, v in out32.items()}
+    d2h_bytes = sum(v.numel() * v.element_size() for v in out32.values())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.celem(out32, hp3d_amd.FP32)
+        for k, v in out32.items():
+            host[k].copy_(v, non_blocking=True)
+    torch.cuda.synchronize()
+    se2e = (time.perf_counter() - t0) / args.steps
+    del host
+
     # BF16 trial: bfloat16 operands, FP32 accumulation, block-tiled element
     # kernel then the blocked condensation
     out16 = batch.alloc_outputs(hp3d_amd.BF16)
@@ -322,6 +343,10 @@ def main():
                              "elem_kernel_ms": e32,
                              "bins": [list(b) + [p] for b, p in zip(batch.bins, batch.bin_paths)]},
             "batched_fp32_graph": {"element_matrices_per_s": nel / sg, "ms_per_step": sg * 1e3},
+            "batched_fp32_end_to_end": {"element_matrices_per_s": nel / se2e, "ms_per_step": se2e * 1e3,
+                                        "d2h_bytes": d2h_bytes,
+                                        "note": "step + D2H of K, load, Aii, Bi, ASchur, BSchur "
+                                                "into pinned host memory"},
             "batched_fp64": {"element_matrices_per_s": nel / s64, "ms_per_step": s64 * 1e3,
                              "elem_kernel_ms": e64,
                              "condensed_dof_per_s": nb_total / s64},
@@ -340,9 +365,12 @@ def main():
     parts = []
     for (lo, cnt, mnd, mnb), path in zip(batch.bins, batch.bin_paths):
         if path == "fused":
-            parts.append("elem_single_kernel<float> (element + condensation, %d elements)" % cnt)
+            r = 64 if mnd <= 64 and mnb <= 16 else 128
+            parts.append("elem_single_kernel<float, %d rows> (element + condensation, %d elements)"
+                         % (r, cnt))
         else:
-            ek = "elem_single_kernel<float>" if mnd <= 128 else "elem_kernel<float>"
+            ek = ("elem_single_kernel<float>" if mnd <= 128 else
+                  "geometry_kernel + elem_kernel<float>")
             parts.append("%s + stc_blocked_kernel<float> (%d elements)" % (ek, cnt))
     kname = " ; ".join(parts)
     achieved = (fk + fc) / (e32 * 1e-3)
