@@ -1202,7 +1202,7 @@ __global__ __launch_bounds__(WG, 2) void stc_blocked_kernel(StcArgs a)
             aii[j * ldc + i] = (double)K[j * ldk + i];
         }
         for (int i = tid; i < ni; i += WG) bi[i] = b[i];
-        if (tid == 0 && a.status) a.status[e] = 0;
+        if (tid == 0 && a.status) a.status[e] = a.status[e] == HP3D_NEG_JACOBIAN ? HP3D_NEG_JACOBIAN : 0;
         return;
     }
     if (ni == 0) {
@@ -1456,7 +1456,9 @@ __global__ __launch_bounds__(WG, 2) void stc_blocked_kernel(StcArgs a)
         X[(size_t)c * nb + r] = Y[(size_t)r * ldy + c];
     }
     for (int r = tid; r < nb; r += WG) a.bschur[a.off_v[e] + r] = Y[(size_t)r * ldy + ni];
-    if (tid == 0 && a.status) a.status[e] = s_fail ? HP3D_E_FACTOR : 0;
+    // the element pass's negative-Jacobian warning survives a successful factorisation
+    if (tid == 0 && a.status)
+        a.status[e] = s_fail ? HP3D_E_FACTOR : (a.status[e] == HP3D_NEG_JACOBIAN ? HP3D_NEG_JACOBIAN : 0);
 }
 
 // panel buffer lengths (in doubles) of the two instantiations: nb up to 144

@@ -164,6 +164,8 @@ size_t hp3d_gpu_elem_workspace(int space, int precision, int nel, int max_ndof);
  *   forward-substituted interface columns)
  *   max_ndof, max_nbubble bound N_e and nb_e over the batch (an element above
  *   them reports HP3D_E_ARG in status[e])
+ *   status[e]: 0, a negative code, or 1 when status[e] already held 1 (the
+ *   negative-Jacobian warning of the element pass that produced kin)
  */
 int hp3d_gpu_stc_batch(int space, int precision, int nel, const int *norder,
                        const long long *off_k, int ld_k, const void *kin,

@@ -319,3 +319,84 @@ def test_small_fused_kernel(gpu, space, pmax):
             assert rel(small.aii_of(o1, e), big.aii_of(o2, e)) < tol
             assert rel(small.vec_of(o1, "bi", e, ni), big.vec_of(o2, "bi", e, ni)) < tol
             assert rel(small.aschur_of(o1, e), big.aschur_of(o2, e)) < (tol if prec == 0 else 1e-3)
+
+
+@pytest.mark.parametrize("porder", [2, 5])
+def test_status_codes_on_device(gpu, porder):
+    """Failure paths of the fused (p = 2) and two-pass (p = 5) bins, forced from
+    the host data (every kernel runs to its end):
+    - an inverted element warns (status 1, geom3D.F90:93-110) when only K is
+      asked for; its K is NaN, as the reference's sqrt(rjac*wa) makes it
+      (elem_opt.F90:101,122), so its condensation fails the SPD check;
+    - kappa_e = 0 makes the bubble block singular (HP3D_E_FACTOR,
+      stc.F90:360-396);
+    - an order code above MAXP given straight to the C ABI reports HP3D_E_ORDER
+      for that element only."""
+    import ctypes
+
+    import torch
+
+    import hp3d_amd
+    from hp3d_amd import _lib
+
+    m = mesh.structured((2, 2, 2), porder=porder, perturb=0.0, mesh_seed=1, perm_seed=2)
+    xnod = m.xnod().copy()
+    xnod[3, :, 0] *= -1.0          # element 3 mirrored in x: det J < 0
+    coef = np.ones(m.nel)
+    coef[5] = 0.0                  # element 5: K = 0, Abb singular
+    batch = hp3d_amd.ElementBatch(0, m.norder, m.nori_e, m.nori_f, xnod, coef)
+    assert batch.bin_paths == (["fused"] if porder == 2 else ["two_pass"])
+    ok = [e for e in range(m.nel) if e not in (3, 5)]
+    ref = oracle.element_loop(0, m.norder[ok], m.nori_e[ok], m.nori_f[ok], xnod[ok], coef[ok],
+                              oracle.LOAD_SINSINSIN, ldk=int(batch.n.max()), herm=True)
+    # K only: the warning, and the other elements exact
+    out = batch.run(hp3d_amd.FP64, want_stc=False)
+    torch.cuda.synchronize()
+    st = batch.status.cpu().numpy()[batch.iperm]
+    assert st[3] == _lib.NEG_JACOBIAN and all(st[e] == 0 for e in range(m.nel) if e != 3), st
+    assert np.isnan(batch.k_of(out, 3)).any()
+    for i, e in enumerate(ok):
+        n = int(batch.n[e])
+        assert rel(batch.k_of(out, e), ref["k"][i].T[:n, :n]) < TOL64
+    # with condensation both forced failures are fatal, the rest condensed
+    out = batch.run(hp3d_amd.FP64)
+    torch.cuda.synchronize()
+    st = batch.status.cpu().numpy()[batch.iperm]
+    assert st[3] == _lib.E_FACTOR and st[5] == _lib.E_FACTOR and all(st[e] == 0 for e in ok), st
+    with pytest.raises(hp3d_amd.Hp3dError):
+        batch.check_status()
+    for i, e in enumerate(ok):
+        ni = int(batch.ni[e])
+        assert rel(batch.aii_of(out, e), ref["aii"][i].T[:ni, :ni]) < TOL64
+
+    # an order code above MAXP, through the C ABI: only that element fails
+    lib = _lib.load()
+    no = np.ascontiguousarray(m.norder, dtype=np.int32).copy()
+    no[2, 0] = 7
+    n, nb = [np.asarray(v) for v in hp3d_amd.ndof(0, m.norder)]
+    dev = torch.device("cuda")
+    t = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt).to(dev)
+    off_v = np.concatenate([[0], np.cumsum(n)[:-1]])
+    off_k = np.concatenate([[0], np.cumsum(n * n)[:-1]])
+    off_c = np.concatenate([[0], np.cumsum((n - nb) ** 2)[:-1]])
+    off_s = np.concatenate([[0], np.cumsum(nb * (n - nb))[:-1]])
+    d_no, d_oe, d_of = t(no, torch.int32), t(m.nori_e, torch.int32), t(m.nori_f, torch.int32)
+    d_x, d_c = t(m.xnod(), torch.float64), t(np.ones(m.nel), torch.float64)
+    d_ok, d_ov, d_oc, d_os = (t(a, torch.int64) for a in (off_k, off_v, off_c, off_s))
+    k = torch.zeros(int((n * n).sum()), dtype=torch.float64, device=dev)
+    b, bi, bs = (torch.zeros(int(n.sum()), dtype=torch.float64, device=dev) for _ in range(3))
+    aii = torch.zeros(int(((n - nb) ** 2).sum()), dtype=torch.float64, device=dev)
+    asc = torch.zeros(max(int((nb * (n - nb)).sum()), 1), dtype=torch.float64, device=dev)
+    mnd, mnb = int(n.max()), int(nb.max())
+    ws = torch.zeros(int(lib.hp3d_gpu_celem_workspace(0, 0, m.nel, mnd, mnb)) // 8 + 1,
+                     dtype=torch.float64, device=dev)
+    status = torch.full((m.nel,), 99, dtype=torch.int32, device=dev)
+    p = lambda x: ctypes.c_void_p(x.data_ptr())
+    rc = lib.hp3d_gpu_celem_batch(0, 0, m.nel, p(d_no), p(d_oe), p(d_of), p(d_x), m.xnod().shape[1],
+                                  p(d_c), 1, p(d_ok), 0, p(k), p(d_ov), p(b), p(d_oc), p(aii), p(bi),
+                                  p(d_os), p(asc), p(bs), p(ws), p(status), mnd, mnb,
+                                  ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+    assert rc == 0
+    torch.cuda.synchronize()
+    s = status.cpu().numpy()
+    assert s[2] == _lib.E_ORDER and all(s[e] == 0 for e in range(m.nel) if e != 2), s
