@@ -724,7 +724,6 @@ __global__ __launch_bounds__(WGE, 4) void elem_kernel(ElemArgs a)
 // The A chunks are double buffered: the MFMAs of chunk c are issued before
 // the VALU/LDS work that builds chunk c+1, so the two overlap.
 constexpr int NBF = 32;      // largest bubble block condensed in the fused path
-constexpr int LDS1 = BR + 16; // LDS stride of the single-block kernel's A columns (see LDA)
 struct CondArgs {
     const long long *off_c, *off_s;
     double *aii, *bi, *aschur, *bschur;
