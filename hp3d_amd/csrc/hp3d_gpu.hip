@@ -675,9 +675,18 @@ __global__ __launch_bounds__(WGE, 4) void elem_kernel(ElemArgs a)
         build(g0, 0, gend, ac);
         __syncthreads();
         for (int c = 0; c < nch; ++c) {
-            mfma(ac + (NBUF == 2 ? (c & 1) * BUFE : 0));
             if constexpr (NBUF == 2) {
-                if (c + 1 < nch) build(g0, (c + 1) * QC, gend, ac + ((c + 1) & 1) * BUFE);
+                // waves w and w + 4 share a SIMD: one issues its MFMAs while the
+                // other builds the next chunk, then they swap
+                T *cur = ac + (c & 1) * BUFE, *nxt = ac + ((c + 1) & 1) * BUFE;
+                const bool first = wv >= 4 && c + 1 < nch, second = wv < 4 && c + 1 < nch;
+                if (first) build(g0, (c + 1) * QC, gend, nxt);
+                __asm__ volatile("" ::: "memory");
+                mfma(cur);
+                __asm__ volatile("" ::: "memory");
+                if (second) build(g0, (c + 1) * QC, gend, nxt);
+            } else {
+                mfma(ac);
             }
             if ((++nfold & 3) == 0) { // FP32: every 4 chunks of 24 columns
 #pragma unroll
