@@ -881,8 +881,40 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
     // unguarded loop keeps the MFMAs back to back.
     const unsigned vmask = __builtin_amdgcn_readfirstlane(valid);
     const bool all_live = vmask == (1u << NSLOT) - 1;
+    // FP32 H1 with all of a wave's tiles inside N: the loop specialised per
+    // wave, so each slot's operand rows are compile-time offsets (no per-MFMA
+    // select); the FP64 and H(curl) forms would spill with it
+    constexpr bool SPEC = sizeof(T) == 4 && SPACE == 0;
+    // wave W's slots (as set above): rows tiA / tiB, first columns tjA / tjB,
+    // nA slots in row tiA, ns slots in all
+    auto mfma_w = [&](auto wc, const T *buf) {
+        constexpr int W = decltype(wc)::value, P = W & 1;
+        constexpr int TA = R == BR ? W : P, TB = R == BR ? 7 - W : 3 - P;
+        constexpr int JA = R == BR ? W : (W < 2 ? P : P + 3), JB = TB;
+        constexpr int NA = R == BR ? 8 - W : (W < 2 ? 3 : 1 - P), NS = R == BR ? 9 : (W < 2 ? 3 : 2);
+#pragma unroll
+        for (int k0 = 0; k0 < KCS; k0 += 4) {
+            const T *col = buf + (k0 + (lane >> 4)) * LD1 + (lane & 15);
+            const T aA = col[TA * 16], aB = col[TB * 16];
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+                acc[s] = M::mma(s < NA ? aA : aB, col[(s < NA ? JA + s : JB + (s - NA)) * 16], acc[s]);
+        }
+    };
+    const int wv = __builtin_amdgcn_readfirstlane(wave);
+    // every slot of this wave holds a tile inside N
+    const bool wave_live = vmask == (1u << nslot) - 1;
     auto mfma = [&](const T *buf) {
-        if (all_live) {
+        if (SPEC && wave_live) {
+            if constexpr (SPEC) {
+                switch (wv) {
+                case 0: mfma_w(std::integral_constant<int, 0>{}, buf); break;
+                case 1: mfma_w(std::integral_constant<int, 1>{}, buf); break;
+                case 2: mfma_w(std::integral_constant<int, 2>{}, buf); break;
+                default: mfma_w(std::integral_constant<int, 3>{}, buf); break;
+                }
+            }
+        } else if (all_live) {
 #pragma unroll
             for (int k0 = 0; k0 < KCS; k0 += 4) {
                 const T *col = buf + (k0 + (lane >> 4)) * LD1;
