@@ -758,13 +758,12 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
     constexpr int NBX = R == BR ? NBF : NBF / 2; // largest bubble block condensed here
     constexpr int LW = NBX + 1;                // leading dimension of L_u^-1
     constexpr int NSLOT = R == BR ? 9 : 3;     // tile slots per wave
-    constexpr int SH = WG / R;                 // threads per row, each with its share of the points
     constexpr int UMAX = R == BR ? 5 : 2;      // right-hand-side tiles per wave in the condensation
     constexpr int BUF = KCS * LD1;             // one A buffer, in T
     constexpr int AC_D = (2 * BUF * (int)sizeof(T) + 7) / 8;
-    constexpr int CW_D = NBX * LD1 + NBX * LW + NBX + LD1 + 4 * 16 * 17 + (SH - 1) * R; // output / condensation workspace
+    constexpr int CW_D = NBX * LD1 + NBX * LW + NBX + LD1 + 4 * 16 * 17; // output / condensation workspace
     constexpr int SM_D = AC_D > CW_D ? AC_D : CW_D;
-    static_assert(GQ % QC == 0 && QC % SH == 0, "geometry passes hold whole chunks");
+    static_assert(GQ % QC == 0 && AC_D >= QC * R, "geometry passes hold whole chunks");
 
     __shared__ ElemShared es;
     __shared__ double tv[3][NTAB * NPT], td[3][NTAB * NPT];
@@ -804,9 +803,8 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
     __syncthreads();
 
     const double coef = a.coef[e];
-    // this thread's row of A and its share sub of each chunk's points
+    // this thread's row of K's outputs (threads with sub == 0 hold one row each)
     const int row = tid & (R - 1), sub = tid / R;
-    const unsigned drow = dsp[row];
     const bool row_ok = row < n;
     // tile slots: s < nA in tile row tiA (tile columns tjA + s), the rest in
     // tile row tiB (tile columns tjB + s - nA).  128 rows: wave w holds tile
@@ -858,22 +856,36 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
         slot_tile(s, ti, tj);
         valid |= ((s < nslot && ti * 16 < n && tj * 16 < n) ? 1u : 0u) << s;
     }
-    double bsum = 0.0;
+    // A chunk build: thread t takes point qt of the chunk and rows
+    // rt + TPP * i (i < RPT), so the point's geometry is read once for all of
+    // them; the load vector's partial sums per row meet after the main loop
+    constexpr int TPP = WG / QC, RPT = R / TPP;
+    static_assert(WG % QC == 0 && R % TPP == 0, "build split");
+    const int qt = tid / TPP, rt = tid % TPP;
+    double bacc[RPT];
+#pragma unroll
+    for (int i = 0; i < RPT; ++i) bacc[i] = 0.0;
 
     auto build = [&](int g0, int l0, int gend, T *buf) {
+        const int lq = l0 + qt;
+        const bool pok = lq < gend;
+        Pt p{0, 0, 0};
+        double g[GS];
+        if (pok) {
+            const unsigned pk = pix[g0 + lq];
+            p = Pt{(int)(pk & 15), (int)((pk >> 4) & 15), (int)(pk >> 8)};
 #pragma unroll
-        for (int qq = 0; qq < QC / SH; ++qq) {
-            const int q = SH * qq + sub, lq = l0 + q;
+            for (int k = 0; k < GS; ++k) g[k] = geo[lq * GS + k];
+        }
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) {
+            const int r = rt + TPP * i;
             double out[NC];
 #pragma unroll
             for (int c = 0; c < NC; ++c) out[c] = 0.0;
-            if (row_ok && lq < gend) {
-                unsigned pk = pix[g0 + lq];
-                Pt p{(int)(pk & 15), (int)((pk >> 4) & 15), (int)(pk >> 8)};
-                bsum += a_entries_load<SPACE>(drow, p, tv, td, geo + lq * GS, out);
-            }
+            if (pok && r < n) bacc[i] += a_entries_load<SPACE>(dsp[r], p, tv, td, g, out);
 #pragma unroll
-            for (int c = 0; c < NC; ++c) buf[(NC * q + c) * LD1 + row] = (T)out[c];
+            for (int c = 0; c < NC; ++c) buf[(NC * qt + c) * LD1 + r] = (T)out[c];
         }
     };
     // The slot mask is wave-uniform.  When some of this wave's tiles lie beyond
@@ -968,13 +980,18 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
     double *rd = wm + NBX * LW;     // 1 / D(j)
     double *bv = rd + NBX;          // load vector
     double *stg = bv + LD1;         // 4 x 16 x 17 wave stages
-    double *bp = stg + 4 * 16 * 17; // partial load sums of shares 1..SH-1
     T *kstage = reinterpret_cast<T *>(stg + wave * (16 * 17));
-    if (sub > 0) bp[(sub - 1) * R + row] = bsum;
-    __syncthreads();
-    if (sub == 0)
+    // the load vector: the QC partial sums of each row (in the free A buffers)
+    double bsum = 0.0;
+    {
+        double *bq = smem; // QC x R
 #pragma unroll
-        for (int k = 0; k < SH - 1; ++k) bsum += bp[k * R + row];
+        for (int i = 0; i < RPT; ++i) bq[qt * R + rt + TPP * i] = bacc[i];
+        __syncthreads();
+        if (sub == 0)
+#pragma unroll
+            for (int q = 0; q < QC; ++q) bsum += bq[q * R + row];
+    }
     const int nb = es.nb, ni = n - nb;
     const bool cond = cd.aii != nullptr && nb <= NBX;
     __syncthreads();
