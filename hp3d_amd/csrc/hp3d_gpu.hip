@@ -271,6 +271,31 @@ __device__ __forceinline__ bool point_geometry(const ElemShared &es, int l, cons
     return det < 0.0;
 }
 
+// for the H(curl) component cp (0..2) with sign sg and the reference gradient
+// gr of the scalar factor: cu = curl of (factor e_cp) = gr x e_cp scaled by sg,
+// mg = row cp of the mass factor gp[9..17] and fl = gp[18 + cp].  With LOCAL
+// (gp a local copy) no array is indexed by the run-time component: 0/1
+// weights instead of a selected address keep gp in registers (its entries are
+// finite); with gp in LDS one indexed load is cheaper.
+template <bool LOCAL>
+__device__ __forceinline__ void curl_mass(int cp, double sg, const double gr[3], const double *gp,
+                                          double cu[3], double mg[3], double &fl)
+{
+    cu[0] = cp == 0 ? 0.0 : (cp == 1 ? -sg * gr[2] : sg * gr[1]);
+    cu[1] = cp == 0 ? sg * gr[2] : (cp == 1 ? 0.0 : -sg * gr[0]);
+    cu[2] = cp == 0 ? -sg * gr[1] : (cp == 1 ? sg * gr[0] : 0.0);
+    if (LOCAL) {
+        const double w0 = cp == 0 ? 1.0 : 0.0, w1 = cp == 1 ? 1.0 : 0.0, w2 = cp == 2 ? 1.0 : 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) mg[c] = w0 * gp[9 + c] + w1 * gp[12 + c] + w2 * gp[15 + c];
+        fl = w0 * gp[18] + w1 * gp[19] + w2 * gp[20];
+    } else {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) mg[c] = gp[9 + 3 * cp + c];
+        fl = gp[18 + cp];
+    }
+}
+
 // the NC entries of row d (descriptor) at point p of the A matrix
 // (A_TEST of elem_opt.F90:122, or the [curl | mass] pair of the H(curl) form)
 template <int SPACE>
@@ -288,22 +313,19 @@ __device__ __forceinline__ void a_entries(unsigned d, Pt p, const double (*tv)[N
         int cp = desc_comp(d);
         double sg = desc_neg(d) ? -1.0 : 1.0;
         double val = sg * v0 * v1 * v2;
-        // curl of val e_cp in reference coordinates: grad(val) x e_cp
-        double cu[3];
-        cu[cp] = 0.0;
-        cu[(cp + 1) % 3] = sg * gr[(cp + 2) % 3];
-        cu[(cp + 2) % 3] = -sg * gr[(cp + 1) % 3];
+        double cu[3], mg[3], fl;
+        curl_mass<false>(cp, sg, gr, gp, cu, mg, fl);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             out[c] = gp[3 * c] * cu[0] + gp[3 * c + 1] * cu[1] + gp[3 * c + 2] * cu[2];
-            out[3 + c] = val * gp[9 + 3 * cp + c];
+            out[3 + c] = val * mg[c];
         }
     }
 }
 
 // A entries and the load contribution of row d at one point (one pass over
 // the 1D tables for both)
-template <int SPACE>
+template <int SPACE, bool LOCAL = false>
 __device__ __forceinline__ double a_entries_load(unsigned d, Pt p, const double (*tv)[NTAB * NPT],
                                                  const double (*td)[NTAB * NPT], const double *gp,
                                                  double *out)
@@ -322,16 +344,14 @@ __device__ __forceinline__ double a_entries_load(unsigned d, Pt p, const double 
         int cp = desc_comp(d);
         double sg = desc_neg(d) ? -1.0 : 1.0;
         double val = sg * ph;
-        double cu[3];
-        cu[cp] = 0.0;
-        cu[(cp + 1) % 3] = sg * gr[(cp + 2) % 3];
-        cu[(cp + 2) % 3] = -sg * gr[(cp + 1) % 3];
+        double cu[3], mg[3], fl;
+        curl_mass<LOCAL>(cp, sg, gr, gp, cu, mg, fl);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             out[c] = gp[3 * c] * cu[0] + gp[3 * c + 1] * cu[1] + gp[3 * c + 2] * cu[2];
-            out[3 + c] = val * gp[9 + 3 * cp + c];
+            out[3 + c] = val * mg[c];
         }
-        return val * gp[18 + cp];
+        return val * fl;
     }
 }
 
@@ -877,13 +897,13 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
 #pragma unroll
             for (int k = 0; k < GS; ++k) g[k] = geo[lq * GS + k];
         }
-#pragma unroll
+#pragma unroll 2
         for (int i = 0; i < RPT; ++i) {
             const int r = rt + TPP * i;
             double out[NC];
 #pragma unroll
             for (int c = 0; c < NC; ++c) out[c] = 0.0;
-            if (pok && r < n) bacc[i] += a_entries_load<SPACE>(dsp[r], p, tv, td, g, out);
+            if (pok && r < n) bacc[i] += a_entries_load<SPACE, true>(dsp[r], p, tv, td, g, out);
 #pragma unroll
             for (int c = 0; c < NC; ++c) buf[(NC * qt + c) * LD1 + r] = (T)out[c];
         }
