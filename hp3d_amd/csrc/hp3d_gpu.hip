@@ -407,6 +407,19 @@ __device__ __forceinline__ void wave_sync_lds()
     __builtin_amdgcn_wave_barrier();
 }
 
+// Workgroup barrier for LDS traffic only: the same LDS-counter wait, then
+// s_barrier, without the workgroup-scope fence of __syncthreads() that would
+// also wait for every outstanding global store.  For phases whose threads
+// share data through LDS alone (their global writes are outputs nobody in the
+// workgroup reads back).
+__device__ __forceinline__ void lds_barrier()
+{
+    __asm__ volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0xC07F);
+    __builtin_amdgcn_s_barrier();
+    __asm__ volatile("" ::: "memory");
+}
+
 // write an accumulator tile (rows R.., cols C..) of a symmetric matrix into
 // column-major `out` (leading dimension ld, valid extent lim x lim), both the
 // tile and its mirror; `stage` is this wave's 16 x 17 LDS scratch.
@@ -994,7 +1007,9 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
     }
 
     // ---- outputs: load vector, K, and the condensed system ----
-    // (the main loop ended on a barrier: the A buffers are free)
+    // (the main loop ended on a barrier: the A buffers are free; from here on
+    // the threads share data through LDS only, and the barriers do not wait
+    // for the K, Aii and ASchur stores)
     double *kb = smem;              // bubble rows b < nb: Abi cols 0..ni-1, Abb cols ni..n-1, Bb col n
     double *wm = kb + NBX * LD1;    // L_u^-1, leading dimension LW
     double *rd = wm + NBX * LW;     // 1 / D(j)
@@ -1007,14 +1022,14 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
         double *bq = smem; // QC x R
 #pragma unroll
         for (int i = 0; i < RPT; ++i) bq[qt * R + rt + TPP * i] = bacc[i];
-        __syncthreads();
+        lds_barrier();
         if (sub == 0)
 #pragma unroll
             for (int q = 0; q < QC; ++q) bsum += bq[q * R + row];
     }
     const int nb = es.nb, ni = n - nb;
     const bool cond = cd.aii != nullptr && nb <= NBX;
-    __syncthreads();
+    lds_barrier();
     if (sub == 0 && row_ok) {
         if (a.bout) a.bout[a.off_v[e] + row] = bsum;
         bv[row] = bsum;
@@ -1055,7 +1070,7 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
             }
         }
     }
-    __syncthreads();
+    lds_barrier();
     if (a.kout) write_k();
     // Gaussian elimination without pivoting of [Abb | I] (Abb = L_u D L_u^T,
     // SPD: every pivot D(j) > 0): the Abb block becomes U = D L_u^T and the
@@ -1077,7 +1092,7 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
             if (c > j) kb[i * LD1 + ni + c] -= m * kb[j * LD1 + ni + c];
             else li[i * LW + c] -= m * li[j * LW + c];
         }
-        __syncthreads();
+        lds_barrier();
     }
 
     // GEMMs against the right-hand block [Abi | Bb] on FP64 MFMA.  Output
@@ -1118,9 +1133,9 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
     };
     d4_t yt[UMAX];
     if (nb > 0) gemm_li(false, yt);
-    __syncthreads();
+    lds_barrier();
     if (nb > 0) store_rhs(yt);
-    __syncthreads();
+    lds_barrier();
     // Z now sits in the bubble rows.  Bi = b_i - Z^T D^-1 z_b
     double *aii = cd.aii + cd.off_c[e];
     double *bi = cd.bi + a.off_v[e];
@@ -1161,9 +1176,9 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
     }
     // X = L_u^-T D^-1 Z = Abb^-1 [Abi | Bb]
     if (nb > 0) gemm_li(true, yt);
-    __syncthreads();
+    lds_barrier();
     if (nb > 0) store_rhs(yt);
-    __syncthreads();
+    lds_barrier();
     // ASchur (nb x ni, leading dimension nb) and BSchur, coalesced
     double *xs = cd.aschur + cd.off_s[e];
     for (int idx = tid; idx < nb * ni; idx += WG) xs[idx] = kb[(idx % nb) * LD1 + idx / nb];
