@@ -582,9 +582,12 @@ __global__ __launch_bounds__(WGE, 4) void elem_kernel(ElemArgs a)
     if (I * BR >= n || J * BR >= n) return; // block beyond this element's size
 
     if (wave < 3) fill_tables(es.nq[wave], tv[wave], td[wave]);
-    for (int r = tid; r < 2 * BR; r += WGE) {
-        int g = (r < BR ? I * BR + r : J * BR + r - BR);
-        dsp[r] = g < n ? (SPACE == 0 ? desc_h1(g, es.o) : desc_hcurl(g, es.o)) : 0u;
+    {
+        const ElemOrders ol = es.o; // in registers for the descriptor walks
+        for (int r = tid; r < 2 * BR; r += WGE) {
+            int g = (r < BR ? I * BR + r : J * BR + r - BR);
+            dsp[r] = g < n ? (SPACE == 0 ? desc_h1(g, ol) : desc_hcurl(g, ol)) : 0u;
+        }
     }
     for (int l = tid; l < nint; l += WGE) {
         Pt p = point_ix(l, es.nq);
@@ -819,9 +822,12 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
     if (wave == 0) {
         fill_tables3(es.nq, lane, tv, td);
     } else {
+        // the orders in registers: the descriptor walks over edges and faces
+        // then run without a chain of LDS round trips
+        const ElemOrders ol = es.o;
         for (int r = tid - 64; r < R; r += WG - 64)
-            dsp[r] = r < n ? (SPACE == 0 ? desc_h1(r, es.o) : desc_hcurl(r, es.o)) : 0u;
-        for (int k = tid - 64; k < a.ngeo; k += WG - 64) dgeo[k] = desc_h1(k, es.o);
+            dsp[r] = r < n ? (SPACE == 0 ? desc_h1(r, ol) : desc_hcurl(r, ol)) : 0u;
+        for (int k = tid - 64; k < a.ngeo; k += WG - 64) dgeo[k] = desc_h1(k, ol);
     }
     for (int l = tid; l < nint; l += WG) {
         Pt p = point_ix(l, es.nq);
@@ -1779,8 +1785,9 @@ __global__ __launch_bounds__(WG) void field_kernel(FieldArgs a)
     if (wave < 3) fill_tables(es.nq[wave], tv[wave], td[wave]);
     const double *ue = a.u + a.off_v[e];
     const double *ve = a.mode == HP3D_FIELD_ADJOINT ? a.v + a.off_v[e] : nullptr;
+    const ElemOrders ol = es.o; // in registers for the descriptor walks
     for (int k = tid; k < n; k += WG) {
-        dsc[k] = SPACE == 0 ? desc_h1(k, es.o) : desc_hcurl(k, es.o);
+        dsc[k] = SPACE == 0 ? desc_h1(k, ol) : desc_hcurl(k, ol);
         cu[k] = ue[k];
         if (ve) cv[k] = ve[k];
     }
