@@ -351,7 +351,8 @@ class ElementBatch:
         current torch stream) the fused bins and the two-pass bins run on two
         side streams forked from and joined back to the current stream, so the
         tail of one launch overlaps the other group's work; the bins write
-        disjoint outputs and only the two-pass bins use the workspace."""
+        disjoint outputs and only the two-pass bins use the workspace (so the
+        BF16 trial, which runs every bin two-pass, stays sequential)."""
         import torch
 
         self._ensure_workspace(precision)
@@ -361,7 +362,8 @@ class ElementBatch:
         # against 2.31 ms): the fused kernels pack worse beside each other
         groups = [g for g in ([i for i, p in enumerate(self.bin_paths) if p == "fused"],
                               [i for i, p in enumerate(self.bin_paths) if p == "two_pass"]) if g]
-        if concurrent and stream is None and len(groups) > 1:
+        # under BF16 every bin runs two-pass through the workspace: sequential
+        if concurrent and stream is None and len(groups) > 1 and precision != BF16:
             main = torch.cuda.current_stream(self.device)
             side = getattr(self, "_side", [])
             while len(side) < len(groups):
