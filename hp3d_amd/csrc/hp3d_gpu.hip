@@ -368,7 +368,7 @@ __device__ __forceinline__ double load_entry(unsigned d, Pt p, const double (*tv
 // element header: orders, counts, quadrature sizes (thread 0), plus status on
 // failure.  Returns false (uniformly) if the element is invalid.
 template <int SPACE>
-__device__ bool element_setup(const ElemArgs &a, int e, ElemShared &es)
+__device__ __forceinline__ bool element_setup(const ElemArgs &a, int e, ElemShared &es)
 {
     // the 37 header integers (norder, nori_e, nori_f) in one parallel load
     __shared__ int hdr[40];
@@ -418,6 +418,69 @@ __device__ __forceinline__ void lds_barrier()
     __builtin_amdgcn_s_waitcnt(0xC07F);
     __builtin_amdgcn_s_barrier();
     __asm__ volatile("" ::: "memory");
+}
+
+// A lane's FP64 value read by the whole wave (two 32-bit readlanes; `l` is
+// wave-uniform).
+__device__ __forceinline__ double readlane_f64(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Gaussian elimination without pivoting of [Abb | I] by one wave, in
+// registers (Abb = L_u D L_u^T is SPD: every pivot D(j) > 0).  Lane c < NBX
+// holds column c of Abb padded to NBX x NBX with the identity, lane NBX + c
+// column c of I, so step j is one readlane of the pivot and one of each
+// multiplier numerator from lane j, then a multiply-add per row in every
+// lane: the row-wise elimination kb(i,:) -= kb(i,j) (kb(j,:) / D(j)), with
+// no workgroup barrier.  Abb comes from `kb`
+// (bubble row i at kb + i * ld, Abb columns from ni); the identity half ends
+// as L_u^-1, written to li (rows < nb, leading dimension lw), and
+// D(j)^-1/2 to rs[j].  Returns true when a pivot is not positive (its 1 / D is
+// then taken as 0).
+template <int NBX>
+__device__ __forceinline__ bool eliminate_wave(const double *kb, int ld, int ni, int nb, double *li, int lw,
+                                               double *rs, int lane)
+{
+    static_assert(2 * NBX <= 64, "one lane per column of [Abb | I]");
+    const int c = lane % NBX;
+    const bool left = lane < NBX;
+    double a[NBX];
+#pragma unroll
+    for (int i = 0; i < NBX; ++i)
+        a[i] = (left && i < nb && c < nb) ? kb[i * ld + ni + c] : (i == c ? 1.0 : 0.0);
+    bool bad = false;
+    double rp_own = 0.0; // lane j keeps 1 / D(j)
+#pragma unroll
+    for (int j = 0; j < NBX; ++j) {
+        if (j < nb) {
+            const double piv = readlane_f64(a[j], j);
+            const double rp = piv > 0.0 ? 1.0 / piv : 0.0;
+            bad |= !(piv > 0.0);
+            if (lane == j) rp_own = rp;
+            const double u = a[j] * rp; // this lane's entry of row j over D(j)
+            // rows in blocks of 8 behind one uniform branch each, so a block's
+            // readlanes and multiply-adds overlap; a padding row (i >= nb) has
+            // the multiplier 0, so its multiply-add leaves it as it is
+#pragma unroll
+            for (int i0 = (j + 1) & ~7; i0 < NBX; i0 += 8) {
+                if (i0 < nb) {
+#pragma unroll
+                    for (int i = i0 > j + 1 ? i0 : j + 1; i < i0 + 8; ++i)
+                        a[i] = fma(-readlane_f64(a[i], j), u, a[i]);
+                }
+            }
+        }
+    }
+    if (!left && lane < 2 * NBX)
+#pragma unroll
+        for (int i = 0; i < NBX; ++i)
+            if (i < nb) li[i * lw + c] = a[i];
+    if (lane < nb) rs[lane] = sqrt(rp_own);
+    return bad;
 }
 
 // write an accumulator tile (rows R.., cols C..) of a symmetric matrix into
@@ -763,9 +826,9 @@ __global__ __launch_bounds__(WGE, 4) void elem_kernel(ElemArgs a)
 // stc_fwd_gen, stc.F90:338-509):
 //   Gaussian elimination without pivoting of [Abb | I] (Abb SPD: Abb =
 //   L_u D L_u^T, the pivots D > 0 are the SPD check) gives D and L_u^-1;
-//   then on FP64 MFMA: Z = L_u^-1 [Abi | Bb];  Aii -= Z^T D^-1 Z and
-//   Bi -= Z^T D^-1 z_b on the register tiles (Aib Abb^-1 Abi = Z^T D^-1 Z);
-//   X = L_u^-T D^-1 Z = Abb^-1 [Abi | Bb] = [ASchur | BSchur].
+//   then on FP64 MFMA: S = D^-1/2 L_u^-1 [Abi | Bb];  Aii -= S^T S and
+//   Bi -= S^T s_b on the register tiles (Aib Abb^-1 Abi = S^T S);
+//   X = L_u^-T D^-1/2 S = Abb^-1 [Abi | Bb] = [ASchur | BSchur].
 // The A chunks are double buffered: the MFMAs of chunk c are issued before
 // the VALU/LDS work that builds chunk c+1, so the two overlap.
 constexpr int NBF = 32;      // largest bubble block condensed in the fused path
@@ -1018,8 +1081,8 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
     // for the K, Aii and ASchur stores)
     double *kb = smem;              // bubble rows b < nb: Abi cols 0..ni-1, Abb cols ni..n-1, Bb col n
     double *wm = kb + NBX * LD1;    // L_u^-1, leading dimension LW
-    double *rd = wm + NBX * LW;     // 1 / D(j)
-    double *bv = rd + NBX;          // load vector
+    double *rs = wm + NBX * LW;     // D(j)^-1/2
+    double *bv = rs + NBX;          // load vector
     double *stg = bv + LD1;         // 4 x 16 x 17 wave stages
     T *kstage = reinterpret_cast<T *>(stg + wave * (16 * 17));
     // the load vector: the QC partial sums of each row (in the free A buffers)
@@ -1059,8 +1122,7 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
             a.status[e] = cd.aii != nullptr ? HP3D_E_ARG : (es.negjac ? HP3D_NEG_JACOBIAN : 0);
         return;
     }
-    // bubble rows of K from the register tiles; L_u^-1 starts as the identity
-    for (int idx = tid; idx < NBX * LW; idx += WG) wm[idx] = (idx / LW == idx % LW) ? 1.0 : 0.0;
+    // bubble rows of K from the register tiles
 #pragma unroll
     for (int s = 0; s < NSLOT; ++s) {
         if (!(valid & (1u << s))) continue;
@@ -1077,33 +1139,17 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
         }
     }
     lds_barrier();
-    if (a.kout) write_k();
-    // Gaussian elimination without pivoting of [Abb | I] (Abb = L_u D L_u^T,
-    // SPD: every pivot D(j) > 0): the Abb block becomes U = D L_u^T and the
-    // identity block L_u^-1.  Items (row i, column c) of a step are spread
-    // over the workgroup, NBX columns per row.
+    // Gaussian elimination of [Abb | I] in the registers of wave EW (L_u^-1
+    // and D^-1/2), while the other waves store their tiles of K; EW stores its
+    // own afterwards.  EW is a wave with the fewest tiles.
+    constexpr int EW = 3;
     double *li = wm; // L_u^-1, leading dimension LW (unit lower; zero above)
-    for (int j = 0; j < nb; ++j) {
-        const double piv = kb[j * LD1 + ni + j];
-        const double rp = piv > 0.0 ? 1.0 / piv : 0.0;
-        if (tid == 0) {
-            rd[j] = rp;
-            if (!(piv > 0.0)) s_fail = 1;
-        }
-        const int nit = (nb - j - 1) * NBX;
-        for (int it = tid; it < nit; it += WG) {
-            const int i = j + 1 + (it / NBX), c = it % NBX;
-            if (c >= nb) continue;
-            const double m = kb[i * LD1 + ni + j] * rp;
-            if (c > j) kb[i * LD1 + ni + c] -= m * kb[j * LD1 + ni + c];
-            else li[i * LW + c] -= m * li[j * LW + c];
-        }
-        lds_barrier();
-    }
-
+    if (wv == EW && eliminate_wave<NBX>(kb, LD1, ni, nb, li, LW, rs, lane) && lane == 0) s_fail = 1;
+    if (a.kout) write_k();
+    lds_barrier();
     // GEMMs against the right-hand block [Abi | Bb] on FP64 MFMA.  Output
     // tiles: rows 16*tr.. (bubble index), columns 16*tc.. (ni Abi columns,
-    // then Bb).  Z = L_u^-1 R; X = U^-1 Z = L_u^-T (D^-1 Z).
+    // then Bb).  Z = L_u^-1 R, stored as S = D^-1/2 Z; X = L_u^-T (D^-1/2 S).
     const int ntr = (nb + 15) / 16, nct = (ni + 16) / 16, ntile = ntr * nct;
     auto rhs_col = [&](int c) { return c < ni ? c : (c == ni ? n : -1); };
     auto gemm_li = [&](bool second, d4_t *out) {
@@ -1118,12 +1164,12 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
                 int k = k0 + (lane >> 4);
                 double av = 0.0, bw = 0.0;
                 if (ar < nb && k < nb) av = second ? li[k * LW + ar] : li[ar * LW + k];
-                if (k < nb && pc >= 0) bw = second ? kb[k * LD1 + pc] * rd[k] : kb[k * LD1 + pc];
+                if (k < nb && pc >= 0) bw = second ? kb[k * LD1 + pc] * rs[k] : kb[k * LD1 + pc];
                 out[u] = MD::mma(av, bw, out[u]);
             }
         }
     };
-    auto store_rhs = [&](const d4_t *in) {
+    auto store_rhs = [&](bool first, const d4_t *in) {
 #pragma unroll
         for (int u = 0; u < UMAX; ++u) {
             int t = wave + 4 * u;
@@ -1133,24 +1179,24 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 int b = tr * 16 + MD::row(lane, j);
-                if (b < nb && pc >= 0) kb[b * LD1 + pc] = in[u][j];
+                if (b < nb && pc >= 0) kb[b * LD1 + pc] = first ? in[u][j] * rs[b] : in[u][j];
             }
         }
     };
     d4_t yt[UMAX];
     if (nb > 0) gemm_li(false, yt);
     lds_barrier();
-    if (nb > 0) store_rhs(yt);
+    if (nb > 0) store_rhs(true, yt);
     lds_barrier();
-    // Z now sits in the bubble rows.  Bi = b_i - Z^T D^-1 z_b
+    // S = D^-1/2 Z now sits in the bubble rows.  Bi = b_i - Z^T D^-1 z_b = b_i - S^T s_b
     double *aii = cd.aii + cd.off_c[e];
     double *bi = cd.bi + a.off_v[e];
     if (tid < ni) {
         double sacc = bv[tid];
-        for (int k = 0; k < nb; ++k) sacc -= kb[k * LD1 + tid] * rd[k] * kb[k * LD1 + n];
+        for (int k = 0; k < nb; ++k) sacc -= kb[k * LD1 + tid] * kb[k * LD1 + n];
         bi[tid] = sacc;
     }
-    // Aii = Kii - Z^T D^-1 Z on the register tiles, in FP64
+    // Aii = Kii - Z^T D^-1 Z = Kii - S^T S on the register tiles, in FP64
 #pragma unroll
     for (int s = 0; s < NSLOT; ++s) {
         if (!(valid & (1u << s))) continue;
@@ -1165,11 +1211,11 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
         for (int k0 = 0; k0 < nb; k0 += 4) {
             int k = k0 + (lane >> 4);
             int ra = R0 + (lane & 15), cb = C0 + (lane & 15);
-            double av = (k < nb && ra < ni) ? kb[k * LD1 + ra] * rd[k] : 0.0;
+            double av = (k < nb && ra < ni) ? kb[k * LD1 + ra] : 0.0;
             double bw = (k < nb && cb < ni) ? kb[k * LD1 + cb] : 0.0;
             up = MD::mma(av, bw, up);
         }
-        // up[j] is (Z^T D^-1 Z)(R0 + MD::row(lane,j), C0 + lane%16): bring it to
+        // up[j] is (S^T S)(R0 + MD::row(lane,j), C0 + lane%16): bring it to
         // the accumulator's row order through this wave's stage
         double *st = stg + wave * (16 * 17);
 #pragma unroll
@@ -1180,10 +1226,10 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
         wave_sync_lds();
         write_sym_tile<d4_t, double, M>(da, R0, C0, ni, aii, (size_t)ni, st, lane);
     }
-    // X = L_u^-T D^-1 Z = Abb^-1 [Abi | Bb]
+    // X = L_u^-T D^-1/2 S = Abb^-1 [Abi | Bb]
     if (nb > 0) gemm_li(true, yt);
     lds_barrier();
-    if (nb > 0) store_rhs(yt);
+    if (nb > 0) store_rhs(false, yt);
     lds_barrier();
     // ASchur (nb x ni, leading dimension nb) and BSchur, coalesced
     double *xs = cd.aschur + cd.off_s[e];
