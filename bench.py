@@ -237,8 +237,8 @@ def main():
                     help="cfg4: elements checked by central differences (step 1e-6)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: element work has no cross-GPU exchange here and "
-              "multi-GPU runs are not part of this build (use --gpus 1)", file=sys.stderr)
+        print("bench.py measures one GPU: the element path runs on a single GPU and has no "
+              "cross-GPU exchange (use --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
     import torch
