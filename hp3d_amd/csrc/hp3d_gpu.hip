@@ -430,6 +430,38 @@ __device__ __forceinline__ double readlane_f64(double v, int l)
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
+// The steps of eliminate_wave over the first NR rows of the lane's column
+// (nb <= NR; rows nb.. are padding, whose multiplier 0 leaves them as they
+// are).  Step j updates row j + 1 first and starts the reciprocal of the
+// next pivot, then the remaining rows, with no branch between them, so the
+// division overlaps the row updates.
+template <int NBX, int NR>
+__device__ __forceinline__ bool eliminate_rows(double (&a)[NBX], int nb, int lane, double &rp_own)
+{
+    bool bad = false;
+    const double piv0 = readlane_f64(a[0], 0);
+    double rp = piv0 > 0.0 ? 1.0 / piv0 : 0.0;
+    bad |= nb > 0 && !(piv0 > 0.0);
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+        if (j < nb) {
+            if (lane == j) rp_own = rp;
+            const double u = a[j] * rp; // this lane's entry of row j over D(j)
+            double rpn = 0.0;
+            if (j + 1 < NR) {
+                a[j + 1] = fma(-readlane_f64(a[j + 1], j), u, a[j + 1]);
+                const double pn = readlane_f64(a[j + 1], j + 1);
+                rpn = pn > 0.0 ? 1.0 / pn : 0.0;
+                bad |= j + 1 < nb && !(pn > 0.0);
+            }
+#pragma unroll
+            for (int i = j + 2; i < NR; ++i) a[i] = fma(-readlane_f64(a[i], j), u, a[i]);
+            rp = rpn;
+        }
+    }
+    return bad;
+}
+
 // Gaussian elimination without pivoting of [Abb | I] by one wave, in
 // registers (Abb = L_u D L_u^T is SPD: every pivot D(j) > 0).  Lane c < NBX
 // holds column c of Abb padded to NBX x NBX with the identity, lane NBX + c
@@ -452,29 +484,9 @@ __device__ __forceinline__ bool eliminate_wave(const double *kb, int ld, int ni,
 #pragma unroll
     for (int i = 0; i < NBX; ++i)
         a[i] = (left && i < nb && c < nb) ? kb[i * ld + ni + c] : (i == c ? 1.0 : 0.0);
-    bool bad = false;
     double rp_own = 0.0; // lane j keeps 1 / D(j)
-#pragma unroll
-    for (int j = 0; j < NBX; ++j) {
-        if (j < nb) {
-            const double piv = readlane_f64(a[j], j);
-            const double rp = piv > 0.0 ? 1.0 / piv : 0.0;
-            bad |= !(piv > 0.0);
-            if (lane == j) rp_own = rp;
-            const double u = a[j] * rp; // this lane's entry of row j over D(j)
-            // rows in blocks of 8 behind one uniform branch each, so a block's
-            // readlanes and multiply-adds overlap; a padding row (i >= nb) has
-            // the multiplier 0, so its multiply-add leaves it as it is
-#pragma unroll
-            for (int i0 = (j + 1) & ~7; i0 < NBX; i0 += 8) {
-                if (i0 < nb) {
-#pragma unroll
-                    for (int i = i0 > j + 1 ? i0 : j + 1; i < i0 + 8; ++i)
-                        a[i] = fma(-readlane_f64(a[i], j), u, a[i]);
-                }
-            }
-        }
-    }
+    const bool bad = nb > NBX / 2 ? eliminate_rows<NBX, NBX>(a, nb, lane, rp_own)
+                                  : eliminate_rows<NBX, NBX / 2>(a, nb, lane, rp_own);
     if (!left && lane < 2 * NBX)
 #pragma unroll
         for (int i = 0; i < NBX; ++i)
