@@ -30,7 +30,16 @@ from .solve import Assembly, DofMap, PCGSolver, Solver, back_substitute, field
 
 class AdjointProblem:
     """Holds the mesh-dependent state (element batch, numbering, assembly
-    pattern) and the target field; `objective` and `gradient` do the work."""
+    pattern) and the target field; `objective` and `gradient` do the work.
+
+    The element matrices, Schur factors and coefficients of the latest
+    forward solve live in buffers shared by every call (`out`, `batch.coef`).
+    A state returned by `objective` records its theta and the generation of
+    those buffers; `gradient` on a state whose buffers a later forward has
+    overwritten first recomputes them from the state's theta (the kernels
+    are deterministic, so the restored K and Schur factors are the same
+    bits), so two forwards before a backward still give each its own
+    gradient."""
 
     def __init__(self, mesh, theta_target, load=LOAD_SINSINSIN, precision=FP64,
                  solver="direct", rtol=1e-13):
@@ -42,6 +51,7 @@ class AdjointProblem:
         self.dofmap = DofMap(mesh, H1)
         self.asm = Assembly(self.batch, self.dofmap)
         self.out = self.batch.alloc_outputs(precision, want_k=True, want_stc=True)
+        self._gen = 0
         self.u_target = self._solve_state(np.asarray(theta_target, dtype=np.float64))[0].clone()
 
     def _set_theta(self, theta):
@@ -51,11 +61,18 @@ class AdjointProblem:
                              device=self.batch.device)
         self.batch.coef.copy_(torch.exp(th))
 
-    def _solve_state(self, theta, x0=None):
-        """u(theta): element dof vectors (internal order), plus the solver."""
+    def _element_pass(self, theta):
+        """Element matrices, condensation and coefficients of theta into the
+        shared buffers (a new generation of them)."""
         self._set_theta(theta)
         self.batch.run(self.precision, want_k=True, want_stc=True, out=self.out)
         self.batch.check_status()
+        self._gen += 1
+        return self._gen
+
+    def _solve_state(self, theta, x0=None):
+        """u(theta): element dof vectors (internal order), plus the solver."""
+        self._element_pass(theta)
         vals = self.asm.matrix_values(self.out["aii"])
         b = self.asm.rhs(self.out["bi"])
         if self.solver_kind == "direct":
@@ -69,17 +86,22 @@ class AdjointProblem:
 
     def objective(self, theta, x0=None):
         """J(theta) and the state needed for its gradient."""
+        theta = np.array(theta, dtype=np.float64)   # the state's own copy
         u, solver, x = self._solve_state(theta, x0)
         d = u - self.u_target
         jl, r = field(self.batch, FIELD_L2, d, with_r=True)
         J = 0.5 * float(jl.sum())
-        return J, {"u": u, "r": r, "solver": solver, "x": x}
+        return J, {"u": u, "r": r, "solver": solver, "x": x, "theta": theta, "gen": self._gen}
 
     def gradient(self, state):
         """dJ/dtheta_e for every element, caller order (numpy)."""
         import torch
 
         b = self.batch
+        if state["gen"] != self._gen:
+            # a later forward overwrote K, the Schur factors and coef: restore
+            # them for this state (u, r and the solver are the state's own)
+            state["gen"] = self._element_pass(state["theta"])
         # condense the adjoint right-hand side M (u - u_t) with the stored K
         adj = {"k": self.out["k"], "b": state["r"],
                "aii": torch.empty_like(self.out["aii"]), "bi": torch.empty_like(self.out["bi"]),

@@ -152,7 +152,7 @@ template <> struct Mfma<float> {
 };
 
 struct ElemArgs {
-    int space, nel, ngeo, load, ld_k, max_blocks_1d;
+    int space, nel, ngeo, load, ld_k, max_ndof; // max_ndof: the caller's bound on N_e
     const int *norder, *nori_e, *nori_f;
     const double *xnod, *coef;
     const long long *off_k, *off_v;
@@ -653,6 +653,12 @@ __global__ __launch_bounds__(WGE, 4) void elem_kernel(ElemArgs a)
         if (tid == 0 && blockIdx.y == 0 && a.status) a.status[e] = HP3D_E_ORDER;
         return;
     }
+    // the grid holds the block rows of the caller's max_ndof: an element
+    // above it would be cut short, so it is an argument error
+    if (es.n > a.max_ndof) {
+        if (tid == 0 && blockIdx.y == 0 && a.status) a.status[e] = HP3D_E_ARG;
+        return;
+    }
     const int n = es.n, nint = es.nint;
     if (I * BR >= n || J * BR >= n) return; // block beyond this element's size
 
@@ -890,6 +896,12 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (!element_setup<SPACE>(a, e, es)) {
         if (tid == 0 && a.status) a.status[e] = HP3D_E_ORDER;
+        return;
+    }
+    // an element above the caller's max_ndof (which chose R) is an argument
+    // error: nothing of it is computed
+    if (es.n > a.max_ndof || es.n > R) {
+        if (tid == 0 && a.status) a.status[e] = HP3D_E_ARG;
         return;
     }
     const int n = es.n, nint = es.nint;
@@ -1299,7 +1311,7 @@ constexpr int SB = 16;
 __host__ __device__ inline int rnd16(int x) { return (x + 15) & ~15; }
 
 struct StcArgs {
-    int space, nel, ld_k, ld_c, nbp_max, ldy_max;
+    int space, nel, ld_k, ld_c, nbp_max, ldy_max, n_max, nb_max; // n_max, nb_max: the caller's bounds
     const int *norder; // order codes, or NULL when nsz gives (n, nb) per element
     const int *nsz;
     const long long *off_k, *off_v, *off_c, *off_s;
@@ -1343,6 +1355,10 @@ __global__ __launch_bounds__(WG, 2) void stc_blocked_kernel(StcArgs a)
     const int n = s_n, nb = s_nb;
     if (n < 0) {
         if (tid == 0 && a.status) a.status[e] = HP3D_E_ORDER;
+        return;
+    }
+    if (n > a.n_max || nb > a.nb_max) { // above the caller's declared bounds
+        if (tid == 0 && a.status) a.status[e] = HP3D_E_ARG;
         return;
     }
     const int ni = n - nb;
@@ -2149,6 +2165,7 @@ int hp3d_gpu_init(int device, int max_ndof)
     HIPCHK(hipMalloc(&g.d_off, sizeof(long long)));
     HIPCHK(hipMemset(g.d_off, 0, sizeof(long long)));
     HIPCHK(hipMalloc(&g.d_status, 2 * sizeof(int)));
+    HIPCHK(hipMemset(g.d_status, 0, 2 * sizeof(int)));
     HIPCHK(hipHostMalloc(&g.h_k, nn * sizeof(double)));
     HIPCHK(hipHostMalloc(&g.h_b, max_ndof * sizeof(double)));
     HIPCHK(hipHostMalloc(&g.h_as, nn * sizeof(double)));
@@ -2183,7 +2200,8 @@ int hp3d_gpu_elem_batch(int space, int precision, int nel, const int *norder, co
 {
     if ((space != HP3D_H1 && space != HP3D_HCURL) || (precision != HP3D_FP64 && precision != HP3D_FP32 && precision != HP3D_BF16))
         return HP3D_E_ARG;
-    if (nel < 0 || ngeo < 1 || ngeo > MAXN_H1 || max_ndof < 1 || ld_k < 0) return HP3D_E_ARG;
+    if (nel < 0 || ngeo < 1 || ngeo > MAXN_H1 || max_ndof < 1 || max_ndof > MAXN_E || ld_k < 0)
+        return HP3D_E_ARG;
     if (nel == 0) return 0;
     if (!norder || !nori_e || !nori_f || !xnod || !coef || !off_k || !kout) return HP3D_E_ARG;
     if (!workspace && hp3d_gpu_elem_workspace(space, precision, nel, max_ndof) > 0) return HP3D_E_ARG;
@@ -2195,7 +2213,7 @@ int hp3d_gpu_elem_batch(int space, int precision, int nel, const int *norder, co
     a.ngeo = ngeo;
     a.load = load;
     a.ld_k = ld_k;
-    a.max_blocks_1d = (max_ndof + BR - 1) / BR;
+    a.max_ndof = max_ndof;
     a.norder = norder;
     a.nori_e = nori_e;
     a.nori_f = nori_f;
@@ -2256,6 +2274,8 @@ int hp3d_gpu_stc_batch(int space, int precision, int nel, const int *norder,
     a.ld_c = ld_c;
     a.nbp_max = rnd16(max_nbubble);
     a.ldy_max = rnd16(max_ndof + 1);
+    a.n_max = max_ndof;
+    a.nb_max = max_nbubble;
     a.norder = norder;
     a.nsz = nullptr;
     a.off_k = off_k;
@@ -2287,7 +2307,8 @@ int hp3d_gpu_celem_batch(int space, int precision, int nel, const int *norder, c
 {
     if ((space != HP3D_H1 && space != HP3D_HCURL) || (precision != HP3D_FP64 && precision != HP3D_FP32 && precision != HP3D_BF16))
         return HP3D_E_ARG;
-    if (nel < 0 || ngeo < 1 || ngeo > MAXN_H1 || max_ndof < 1 || max_nbubble < 0 || ld_k < 0)
+    if (nel < 0 || ngeo < 1 || ngeo > MAXN_H1 || max_ndof < 1 || max_ndof > MAXN_E || max_nbubble < 0 ||
+        max_nbubble > max_ndof || ld_k < 0)
         return HP3D_E_ARG;
     if (nel == 0) return 0;
     if (!norder || !nori_e || !nori_f || !xnod || !coef || !off_v || !off_c || !aii || !bi ||
@@ -2301,7 +2322,7 @@ int hp3d_gpu_celem_batch(int space, int precision, int nel, const int *norder, c
     a.ngeo = ngeo;
     a.load = load;
     a.ld_k = ld_k;
-    a.max_blocks_1d = (max_ndof + BR - 1) / BR;
+    a.max_ndof = max_ndof;
     a.norder = norder;
     a.nori_e = nori_e;
     a.nori_f = nori_f;
@@ -2457,6 +2478,8 @@ static int stc_run(int space, int n, int nb, const int *d_norder, const int *d_n
     a.ld_c = 0;
     a.nbp_max = rnd16(g.nbm);
     a.ldy_max = rnd16(g.max_ndof + 1);
+    a.n_max = g.max_ndof;
+    a.nb_max = g.nbm;
     a.norder = d_norder;
     a.nsz = d_nsz;
     a.off_k = a.off_v = a.off_c = a.off_s = g.d_off;
@@ -2469,6 +2492,9 @@ static int stc_run(int space, int n, int nb, const int *d_norder, const int *d_n
     a.ws = g.d_ws;
     a.status = d_status;
     (void)n;
+    // the kernel keeps a negative-Jacobian warning it finds in its status
+    // slot: start from 0 so nothing of an earlier call survives
+    HIPCHK(hipMemsetAsync(d_status, 0, sizeof(int), g.stream));
     launch_stc<double>(a, nb, g.stream);
     HIPCHK(hipGetLastError());
     return 0;

@@ -15,6 +15,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(HERE, "liboracle.so")
 _REF = os.path.join(HERE, "_ref", "libhp3d_ref.so")
+REF_PATH = _REF  # tests check for it without loading the library
 
 MAXP = 6
 MAXBRICKH = 343

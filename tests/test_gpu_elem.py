@@ -1,7 +1,13 @@
 """Parity of the HIP element path (through the C ABI) against the oracle.
 
-Tolerances: FP64 element, load and condensed matrices 1e-12 relative
-Frobenius per element; FP32 contraction 1e-6 relative Frobenius for K.
+Tolerances (relative Frobenius, per element):
+- FP64: K, b, Aii and Bi 1e-12.  ASchur = Abb^-1 Abi and BSchur = Abb^-1 Bb
+  are solutions of the bubble system, whose forward error is bounded by the
+  condition number: both the GPU and the oracle sit within a small multiple
+  of nb cond(Abb) eps of the exact value, so they are held to
+  max(1e-12, nb cond(Abb) eps) with cond(Abb) computed here per element.
+- FP32 contraction (FP64 geometry and condensation): K within 1e-6; the
+  condensed outputs within the stated bounds of each test.
 """
 import numpy as np
 import pytest
@@ -13,6 +19,7 @@ pytestmark = pytest.mark.gpu
 
 TOL64 = 1e-12
 TOL32 = 1e-6
+EPS = np.finfo(np.float64).eps
 
 
 def rel(a, b):
@@ -36,24 +43,42 @@ def run_batch(m, space, precision, want_stc=True, herm=True):
     return batch, out, ref
 
 
-def compare(batch, out, ref, tol_k, tol_c, want_stc=True, key="k"):
+def schur_bound(ref, e, n, nb):
+    """nb cond(Abb) eps for element e (Abb from the oracle's K)."""
+    kr = ref["k"][e].T[:n, :n]
+    return nb * np.linalg.cond(kr[n - nb:, n - nb:]) * EPS
+
+
+def compare(batch, out, ref, tol_k, tol_c, want_stc=True, tol_schur=None, sample=None):
+    """Worst per-element errors of every output against the oracle.
+
+    tol_k bounds K, tol_c bounds Aii and Bi (the load b is always FP64 and
+    held to 1e-12); ASchur and BSchur are held to tol_schur(e, n, nb) when
+    given, else to max(tol_c, nb cond(Abb) eps).  `sample` picks the
+    elements (all by default; ref then holds those elements in that order)."""
     worst = {"k": 0.0, "b": 0.0, "aii": 0.0, "bi": 0.0, "aschur": 0.0, "bschur": 0.0}
-    for e in range(batch.nel):
+    schur_ratio = 0.0   # worst error / bound of ASchur and BSchur
+    elems = range(batch.nel) if sample is None else sample
+    for i, e in enumerate(elems):
         n, nb, ni = int(batch.n[e]), int(batch.nb[e]), int(batch.ni[e])
-        worst["k"] = max(worst["k"], rel(batch.k_of(out, e), ref["k"][e].T[:n, :n]))
-        worst["b"] = max(worst["b"], rel(batch.vec_of(out, "b", e), ref["b"][e][:n]))
+        worst["k"] = max(worst["k"], rel(batch.k_of(out, e), ref["k"][i].T[:n, :n]))
+        worst["b"] = max(worst["b"], rel(batch.vec_of(out, "b", e), ref["b"][i][:n]))
         if want_stc and nb > 0:
-            worst["aii"] = max(worst["aii"], rel(batch.aii_of(out, e), ref["aii"][e].T[:ni, :ni]))
-            worst["bi"] = max(worst["bi"], rel(batch.vec_of(out, "bi", e, ni), ref["bi"][e][:ni]))
-            asr = ref["aschur"][e].reshape(-1)[: nb * ni].reshape(ni, nb).T
-            worst["aschur"] = max(worst["aschur"], rel(batch.aschur_of(out, e), asr))
-            worst["bschur"] = max(worst["bschur"],
-                                  rel(batch.vec_of(out, "bschur", e, nb), ref["bschur"][e][:nb]))
-    print({k: "%.2e" % v for k, v in worst.items()})
-    assert worst["k"] < tol_k and worst["b"] < TOL64
+            worst["aii"] = max(worst["aii"], rel(batch.aii_of(out, e), ref["aii"][i].T[:ni, :ni]))
+            worst["bi"] = max(worst["bi"], rel(batch.vec_of(out, "bi", e, ni), ref["bi"][i][:ni]))
+            asr = ref["aschur"][i].reshape(-1)[: nb * ni].reshape(ni, nb).T
+            ea = rel(batch.aschur_of(out, e), asr)
+            eb = rel(batch.vec_of(out, "bschur", e, nb), ref["bschur"][i][:nb])
+            worst["aschur"] = max(worst["aschur"], ea)
+            worst["bschur"] = max(worst["bschur"], eb)
+            bound = tol_schur(i, n, nb) if tol_schur else max(tol_c, schur_bound(ref, i, n, nb))
+            schur_ratio = max(schur_ratio, ea / bound, eb / bound)
+    print({k: "%.2e" % v for k, v in worst.items()}, "schur error / bound %.3f" % schur_ratio)
+    assert worst["k"] < tol_k and worst["b"] < TOL64, worst
     if want_stc:
-        for k in ("aii", "bi", "aschur", "bschur"):
+        for k in ("aii", "bi"):
             assert worst[k] < tol_c, (k, worst[k])
+        assert schur_ratio < 1.0, schur_ratio
     return worst
 
 
@@ -78,14 +103,21 @@ def test_h1_large_p6_fp64(gpu):
     # p = 6: N = 343 > 128, several K blocks per element
     m = mesh.structured((2, 1, 1), porder=6, perturb=0.1, mesh_seed=1, perm_seed=2)
     batch, out, ref = run_batch(m, 0, 0)
-    compare(batch, out, ref, TOL64, 1e-11)
+    compare(batch, out, ref, TOL64, TOL64)
+
+
+# FP32 contraction: K carries the FP32 rounding of the carried sum (about
+# 1e-7); Aii and Bi stay within 1e-6; ASchur = Abb^-1 Abi and BSchur inherit
+# cond(Abb) times that perturbation and are held to the stated bound of each
+# test (a few times the measured worst case)
+SCHUR32_H1_P4 = 4e-6
 
 
 def test_h1_fp32(gpu):
     m = mesh.structured((3, 3, 2), porder=4, perturb=0.15, mesh_seed=9, perm_seed=9,
                         coef=np.exp(np.random.default_rng(9).normal(size=18)))
     batch, out, ref = run_batch(m, 0, 1)
-    w = compare(batch, out, ref, TOL32, 1e-3)
+    w = compare(batch, out, ref, TOL32, TOL32, tol_schur=lambda i, n, nb: SCHUR32_H1_P4)
     print("fp32 worst", w)
 
 
@@ -94,7 +126,7 @@ def test_hcurl_fp64(gpu, p):
     m = mesh.structured((2, 2, 1), porder=p, perturb=0.1, mesh_seed=20 + p, perm_seed=30 + p,
                         coef=np.random.default_rng(p).uniform(1, 4, 4))
     batch, out, ref = run_batch(m, 1, 0)
-    compare(batch, out, ref, TOL64, 1e-11)
+    compare(batch, out, ref, TOL64, TOL64)
 
 
 @pytest.mark.parametrize("precision", [0, 1])
@@ -172,7 +204,7 @@ def test_order_bins_mixed_mesh_and_graph_replay(gpu):
     batch.check_status()
     ref = oracle.element_loop(0, m.norder, m.nori_e, m.nori_f, m.xnod(), m.coef,
                               oracle.LOAD_SINSINSIN, ldk=int(batch.n.max()), herm=True)
-    compare(batch, out, ref, TOL64, 1e-11)
+    compare(batch, out, ref, TOL64, TOL64)
     eager = {k: v.clone() for k, v in out.items()}
     g = batch.capture(out, 0)
     for rep in range(3):
@@ -242,19 +274,21 @@ def test_hcurl_large_fp64(gpu, p):
     m = mesh.structured((2, 1, 1), porder=p, perturb=0.1, mesh_seed=40 + p, perm_seed=50 + p,
                         coef=np.array([1.5, 3.0]))
     batch, out, ref = run_batch(m, 1, 0)
-    compare(batch, out, ref, TOL64, 1e-10)
+    compare(batch, out, ref, TOL64, TOL64)
+
+
+SCHUR32_HCURL = 2e-5
 
 
 def test_hcurl_fp32_carry(gpu):
     """FP32 contraction with the TwoSum carry on a long k-sum (H(curl) p = 4,
-    k = 6 x 125 = 750 columns): K and the condensed Aii within 1e-6 of the
-    oracle.  ASchur = Abb^-1 Abi inherits cond(Abb) times K's FP32 error and
-    is held to 1e-3, as in the H1 FP32 test."""
+    k = 6 x 125 = 750 columns): K and the condensed Aii and Bi within 1e-6 of
+    the oracle; ASchur and BSchur within the stated H(curl) FP32 bound."""
     m = mesh.structured((2, 2, 1), porder=4, perturb=0.1, mesh_seed=7, perm_seed=8,
                         coef=np.random.default_rng(4).uniform(1, 4, 4))
     batch, out, ref = run_batch(m, 1, 1)
-    w = compare(batch, out, ref, TOL32, 1e-3)
-    assert w["k"] < 5e-7 and w["aii"] < 1e-6 and w["bi"] < 1e-6
+    w = compare(batch, out, ref, TOL32, TOL32, tol_schur=lambda i, n, nb: SCHUR32_HCURL)
+    assert w["k"] < 5e-7
 
 
 @pytest.mark.parametrize("precision", [0, 1])
@@ -309,7 +343,7 @@ def test_small_fused_kernel(gpu, space, pmax):
         small.check_status()
         big.check_status()
         if prec == 0:
-            compare(small, o1, ref, TOL64, 1e-11)
+            compare(small, o1, ref, TOL64, TOL64)
         for e in range(m.nel):
             ni = int(small.ni[e])
             assert rel(small.k_of(o1, e), big.k_of(o2, e)) < tol

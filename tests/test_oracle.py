@@ -64,7 +64,8 @@ def test_fixtures_legendre_gauss_geom_bitwise():
     assert g["geom_flag"].sum() == 1
 
 
-@pytest.mark.skipif(oracle.ref() is None, reason="oracle/_ref not built (needs the reference tree)")
+@pytest.mark.skipif(not os.path.exists(oracle.REF_PATH),
+                    reason="oracle/_ref not built (needs the reference tree)")
 def test_live_reference_shape_functions_bitwise():
     rng = np.random.default_rng(1)
     for t in range(300):
@@ -81,7 +82,8 @@ def test_live_reference_shape_functions_bitwise():
         assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
 
 
-@pytest.mark.skipif(oracle.ref() is None, reason="oracle/_ref not built (needs the reference tree)")
+@pytest.mark.skipif(not os.path.exists(oracle.REF_PATH),
+                    reason="oracle/_ref not built (needs the reference tree)")
 def test_live_reference_tables_bitwise():
     for n in range(1, 11):
         assert all(np.array_equal(u, v) for u, v in zip(oracle.gauss1d(n), oracle.ref_gauss1d(n)))
