@@ -20,7 +20,7 @@ import oracle
 from hp3d_amd import mesh
 
 import solution_ref as sr
-from test_gpu_elem import SCHUR32_HCURL, TOL32, TOL64, compare, rel, run_batch
+from test_gpu_elem import TOL32, TOL64, compare, rel, run_batch
 
 pytestmark = pytest.mark.gpu
 
@@ -36,7 +36,7 @@ def test_hcurl_p5_fp32_and_fp64(gpu):
                         coef=np.random.default_rng(2003).uniform(1.0, 4.0, 4))
     batch, out, ref = run_batch(m, 1, 1)
     assert int(batch.n.max()) == 540 and int(batch.nb.max()) == 240
-    w = compare(batch, out, ref, TOL32, TOL32, tol_schur=lambda i, n, nb: SCHUR32_HCURL)
+    w = compare(batch, out, ref, TOL32, TOL32, schur32=True)
     print("H(curl) p=5 FP32", w)
     batch, out, ref = run_batch(m, 1, 0)
     compare(batch, out, ref, TOL64, TOL64)
@@ -67,8 +67,7 @@ def test_config_mesh_sample(gpu, name):
         if prec == hp3d_amd.FP64:
             compare(batch, out, ref, TOL64, TOL64, sample=sample)
         else:
-            w = compare(batch, out, ref, TOL32, TOL32, sample=sample,
-                        tol_schur=lambda i, n, nb: 4e-6)
+            w = compare(batch, out, ref, TOL32, TOL32, sample=sample, schur32=True)
             print(name, "FP32", w)
         del out
         torch.cuda.empty_cache()
