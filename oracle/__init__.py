@@ -72,6 +72,9 @@ def _load():
         "orc_elem_mass": (_i, [_i, _p, _p, _p, _p, _i, _p, _i]),
         "orc_element_loop": (_i, [_i, _i, _p, _p, _p, _p, _i, _p, _i, _i,
                                   _p, _p, _p, _p, _p, _p, _i, _i]),
+        "orc_set_blas": (_i, [_p, _p, _p, _p]),
+        "orc_element_loop_blas": (_i, [_i, _i, _p, _p, _p, _p, _i, _p, _i, _i,
+                                       _p, _p, _p, _p, _p, _p, _i]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -334,6 +337,69 @@ def element_loop(space, norder, nori_e, nori_f, xnod, coef, load, ldk, want_k=Tr
                               ngeo, _ptr(coef), load, ldk, _ptr(g("k")), _ptr(g("b")),
                               _ptr(g("aii")), _ptr(g("bi")), _ptr(g("aschur")),
                               _ptr(g("bschur")), 1 if herm else 0, nthreads)
+    out["status"] = st
+    return out
+
+
+_blas_name = None
+
+
+def host_blas():
+    """The host BLAS/LAPACK the reference-shaped loop calls: scipy's bundled
+    OpenBLAS (32-bit integers, Fortran ABI, symbols prefixed scipy_), loaded
+    once and handed to orc_set_blas.  Returns a description of the library."""
+    global _blas_name
+    if _blas_name is not None:
+        return _blas_name
+    import glob
+
+    import scipy
+
+    libdir = os.path.join(os.path.dirname(os.path.dirname(scipy.__file__)), "scipy.libs")
+    cands = sorted(glob.glob(os.path.join(libdir, "libscipy_openblas*.so")))
+    if not cands:
+        raise RuntimeError("no host BLAS found under %s" % libdir)
+    blas = ctypes.CDLL(cands[0])
+    addr = lambda n: ctypes.cast(getattr(blas, n), ctypes.c_void_p)
+    blas.scipy_openblas_set_num_threads.argtypes = [_i]
+    blas.scipy_openblas_set_num_threads(1)   # one BLAS thread per OpenMP thread
+    rc = lib.orc_set_blas(addr("scipy_dsyrk_"), addr("scipy_dgemm_"), addr("scipy_dgetrf_"),
+                          addr("scipy_dgetrs_"))
+    if rc != 0:
+        raise RuntimeError("orc_set_blas failed")
+    cfg = getattr(blas, "scipy_openblas_get_config", None)
+    desc = os.path.basename(cands[0])
+    if cfg is not None:
+        cfg.restype = ctypes.c_char_p
+        desc += " (" + cfg().decode().strip() + ")"
+    _blas_name = desc
+    return desc
+
+
+def element_loop_blas(space, norder, nori_e, nori_f, xnod, coef, load, ldk, nthreads=0,
+                      want_k=True, want_stc=True):
+    """The element loop as the reference builds it: per element elem_opt with
+    DSYRK and stc_fwd_gen with DGETRF/DGETRS/DGEMM from the host BLAS, OpenMP
+    dynamic schedule over elements (mumps_sc.F90:224-342).  Outputs as
+    element_loop."""
+    host_blas()
+    norder = np.ascontiguousarray(norder, dtype=np.int32)
+    nel = norder.shape[0]
+    nori_e = np.ascontiguousarray(nori_e, dtype=np.int32)
+    nori_f = np.ascontiguousarray(nori_f, dtype=np.int32)
+    xnod = np.ascontiguousarray(xnod, dtype=np.float64)
+    coef = np.ascontiguousarray(coef, dtype=np.float64)
+    out = {}
+    if want_k:
+        out["k"], out["b"] = np.zeros((nel, ldk, ldk)), np.zeros((nel, ldk))
+    if want_stc:
+        out["aii"], out["bi"] = np.zeros((nel, ldk, ldk)), np.zeros((nel, ldk))
+        out["aschur"], out["bschur"] = np.zeros((nel, ldk, ldk)), np.zeros((nel, ldk))
+    g = out.get
+    st = lib.orc_element_loop_blas(space, nel, _ptr(norder), _ptr(nori_e), _ptr(nori_f),
+                                   _ptr(xnod), xnod.shape[1], _ptr(coef), load, ldk, _ptr(g("k")),
+                                   _ptr(g("b")), _ptr(g("aii")), _ptr(g("bi")),
+                                   _ptr(g("aschur")), _ptr(g("bschur")), nthreads)
     out["status"] = st
     return out
 

@@ -38,6 +38,9 @@
  *       identity (full solve == condensed solve + back substitution).
  *   orc_element_loop
  *       composition of the above; checked element by element.
+ *   orc_element_loop_blas (the CPU baseline's reference-shaped loop)
+ *       the same elements with DSYRK and DGETRF/DGETRS/DGEMM from a host
+ *       BLAS/LAPACK: equal to orc_element_loop to 1e-12 (tests/test_oracle.py).
  *
  * Build: oracle/Makefile (gcc -O2 -ffp-contract=off, no FMA on x86-64
  * baseline, so rounding follows the source order like the reference build).
@@ -795,8 +798,11 @@ static void mirror_lower(int n, double *c, int ldc)
 
 /* elem_opt (Poisson): problems/POISSON/GALERKIN/elem_opt.F90:22-141, with a per-element
  * conductivity kappa folded into the weight: A_TEST = grad(phi) sqrt(kappa w detJ) */
-int orc_elem_h1(const int *norder, const int *nori_e, const int *nori_f, const double *xnod,
-                int ngeo, double kappa, int load, double *zaloc, int ld, double *zbloc)
+static void syrk_blas(int n, int k, const double *a, double beta, double *c, int ldc);
+
+static int elem_h1_core(const int *norder, const int *nori_e, const int *nori_f,
+                        const double *xnod, int ngeo, double kappa, int load, double *zaloc,
+                        int ld, double *zbloc, int blas)
 {
     if (orc_checkorder(norder) < 0) return -1;
     int n = orc_ndof_h1(norder);
@@ -826,17 +832,26 @@ int orc_elem_h1(const int *norder, const int *nori_e, const int *nori_f, const d
             }
         }
     }
-    syrk_upper_add(n, nda, atest, 1.0, zaloc, ld, 0);
+    if (blas)
+        syrk_blas(n, nda, atest, 0.0, zaloc, ld);
+    else
+        syrk_upper_add(n, nda, atest, 1.0, zaloc, ld, 0);
     mirror_lower(n, zaloc, ld);
     free(atest);
     return status;
 }
 
+int orc_elem_h1(const int *norder, const int *nori_e, const int *nori_f, const double *xnod,
+                int ngeo, double kappa, int load, double *zaloc, int ld, double *zbloc)
+{
+    return elem_h1_core(norder, nori_e, nori_f, xnod, ngeo, kappa, load, zaloc, ld, zbloc, 0);
+}
+
 /* elem_opt (Maxwell) real restatement: problems/MAXWELL/GALERKIN/elem_opt.F90:22-158 with
  * K = A A^T + M M^T, A = curl(F) sqrt(w/mu), M = F sqrt(eps w), b = (J, F) */
-int orc_elem_hcurl(const int *norder, const int *nori_e, const int *nori_f, const double *xnod,
-                   int ngeo, double mu, double eps, int load, double *zaloc, int ld,
-                   double *zbloc)
+static int elem_hcurl_core(const int *norder, const int *nori_e, const int *nori_f,
+                           const double *xnod, int ngeo, double mu, double eps, int load,
+                           double *zaloc, int ld, double *zbloc, int blas)
 {
     if (orc_checkorder(norder) < 0) return -1;
     int nh = orc_ndof_h1(norder), n = orc_ndof_hcurl(norder);
@@ -876,14 +891,27 @@ int orc_elem_hcurl(const int *norder, const int *nori_e, const int *nori_f, cons
             }
         }
     }
-    syrk_upper_add(n, nda, at, 1.0, zaloc, ld, 0);
-    syrk_upper_add(n, nda, mt, 1.0, zaloc, ld, 1);
+    if (blas) {
+        syrk_blas(n, nda, at, 0.0, zaloc, ld);
+        syrk_blas(n, nda, mt, 1.0, zaloc, ld);
+    } else {
+        syrk_upper_add(n, nda, at, 1.0, zaloc, ld, 0);
+        syrk_upper_add(n, nda, mt, 1.0, zaloc, ld, 1);
+    }
     mirror_lower(n, zaloc, ld);
     free(at);
     free(mt);
     free(she);
     free(cue);
     return status;
+}
+
+int orc_elem_hcurl(const int *norder, const int *nori_e, const int *nori_f, const double *xnod,
+                   int ngeo, double mu, double eps, int load, double *zaloc, int ld,
+                   double *zbloc)
+{
+    return elem_hcurl_core(norder, nori_e, nori_f, xnod, ngeo, mu, eps, load, zaloc, ld, zbloc,
+                           0);
 }
 
 /* ---------------------------------------------------- static condensation */
@@ -1065,14 +1093,90 @@ void orc_stc_bwd(int ni, int nb, const double *aschur, const double *bschur, con
     }
 }
 
+/* ------------------------------------------ BLAS / LAPACK form (CPU baseline) */
+
+/* Host BLAS and LAPACK entry points (Fortran calling convention, 32-bit
+ * integers), set by orc_set_blas from whatever library the caller found (the
+ * reference links BLAS/LAPACK the same way: elem_opt.F90:131 DSYRK,
+ * stc.F90:464-509 DGETRF/DGETRS/DGEMM).  Used only by orc_element_loop_blas. */
+typedef void (*dsyrk_fn)(const char *, const char *, const int *, const int *, const double *,
+                         const double *, const int *, const double *, double *, const int *,
+                         size_t, size_t);
+typedef void (*dgemm_fn)(const char *, const char *, const int *, const int *, const int *,
+                         const double *, const double *, const int *, const double *,
+                         const int *, const double *, double *, const int *, size_t, size_t);
+typedef void (*dgetrf_fn)(const int *, const int *, double *, const int *, int *, int *);
+typedef void (*dgetrs_fn)(const char *, const int *, const int *, const double *, const int *,
+                          const int *, double *, const int *, int *, size_t);
+static dsyrk_fn g_dsyrk;
+static dgemm_fn g_dgemm;
+static dgetrf_fn g_dgetrf;
+static dgetrs_fn g_dgetrs;
+
+int orc_set_blas(void *dsyrk, void *dgemm, void *dgetrf, void *dgetrs)
+{
+    g_dsyrk = (dsyrk_fn)dsyrk;
+    g_dgemm = (dgemm_fn)dgemm;
+    g_dgetrf = (dgetrf_fn)dgetrf;
+    g_dgetrs = (dgetrs_fn)dgetrs;
+    return (g_dsyrk && g_dgemm && g_dgetrf && g_dgetrs) ? 0 : -5;
+}
+
+/* DSYRK('U','N',n,k,1,A,n,beta,C,ldc): elem_opt.F90:131-132 */
+static void syrk_blas(int n, int k, const double *a, double beta, double *c, int ldc)
+{
+    const double one = 1.0;
+    g_dsyrk("U", "N", &n, &k, &one, a, &n, &beta, c, &ldc, 1, 1);
+}
+
+/* stc_fwd_gen through LAPACK and BLAS, in the reference's call order
+ * (stc.F90:443-509): DGETRF on Abb, DGETRS for Bb then Abi, DGEMM for Bi and
+ * Aii; packing and write-back as stc_fwd_wrapper (stc.F90:227-305) */
+static int stc_fwd_gen_blas(int ni, int nb, double *aloc, int ld, double *bloc, double *aschur,
+                            double *bschur)
+{
+    if (nb == 0) return 0;
+    if (ni == 0) return -3;
+    double *aii = malloc(sizeof(double) * (size_t)ni * ni);
+    double *abi = malloc(sizeof(double) * (size_t)nb * ni);
+    double *aib = malloc(sizeof(double) * (size_t)ni * nb);
+    double *abb = malloc(sizeof(double) * (size_t)nb * nb);
+    double *bi = malloc(sizeof(double) * ni), *bb = malloc(sizeof(double) * nb);
+    int *piv = malloc(sizeof(int) * nb);
+    int info = 0, one = 1;
+    const double mone = -1.0, pone = 1.0;
+    stc_pack(ni, nb, aloc, ld, bloc, aii, abi, aib, abb, bi, bb);
+    g_dgetrf(&nb, &nb, abb, &nb, piv, &info);
+    if (info == 0) g_dgetrs("N", &nb, &one, abb, &nb, piv, bb, &nb, &info, 1);
+    if (info == 0) g_dgetrs("N", &nb, &ni, abb, &nb, piv, abi, &nb, &info, 1);
+    if (info == 0) {
+        g_dgemm("N", "N", &ni, &one, &nb, &mone, aib, &ni, bb, &nb, &pone, bi, &ni, 1, 1);
+        g_dgemm("N", "N", &ni, &ni, &nb, &mone, aib, &ni, abi, &nb, &pone, aii, &ni, 1, 1);
+        for (int j = 0; j < ni; j++)
+            memcpy(&aloc[(size_t)j * ld], &aii[(size_t)j * ni], sizeof(double) * ni);
+        memcpy(bloc, bi, sizeof(double) * ni);
+        if (aschur) memcpy(aschur, abi, sizeof(double) * (size_t)nb * ni);
+        if (bschur) memcpy(bschur, bb, sizeof(double) * nb);
+    }
+    free(aii);
+    free(abi);
+    free(aib);
+    free(abb);
+    free(bi);
+    free(bb);
+    free(piv);
+    return info ? -4 : 0;
+}
+
 /* ------------------------------------------------------------ element loop */
 
 /* mumps_sc Step 2 (omp_mumps/mumps_sc.F90:224-342): !$OMP DO SCHEDULE(DYNAMIC) over
  * elements, each running elem + stc_fwd on thread-private ALOC/BLOC workspaces */
-int orc_element_loop(int space, int nel, const int *norder, const int *nori_e, const int *nori_f,
-                     const double *xnod, int ngeo, const double *coef, int load, int ldk,
-                     double *kout, double *bout, double *aii, double *bi, double *aschur,
-                     double *bschur, int herm, int nthreads)
+static int element_loop_core(int space, int nel, const int *norder, const int *nori_e,
+                             const int *nori_f, const double *xnod, int ngeo, const double *coef,
+                             int load, int ldk, double *kout, double *bout, double *aii,
+                             double *bi, double *aschur, double *bschur, int herm, int nthreads,
+                             int blas)
 {
     int status = 0;
 #ifdef _OPENMP
@@ -1089,14 +1193,15 @@ int orc_element_loop(int space, int nel, const int *norder, const int *nori_e, c
             if (space == 0) {
                 n = orc_ndof_h1(no);
                 nb = orc_nbubble_h1(no[18]);
-                st = orc_elem_h1(no, &nori_e[12 * (size_t)e], &nori_f[6 * (size_t)e],
-                                 &xnod[3 * (size_t)ngeo * e], ngeo, coef[e], load, aloc, ldk, bloc);
+                st = elem_h1_core(no, &nori_e[12 * (size_t)e], &nori_f[6 * (size_t)e],
+                                  &xnod[3 * (size_t)ngeo * e], ngeo, coef[e], load, aloc, ldk,
+                                  bloc, blas);
             } else {
                 n = orc_ndof_hcurl(no);
                 nb = orc_nbubble_hcurl(no[18]);
-                st = orc_elem_hcurl(no, &nori_e[12 * (size_t)e], &nori_f[6 * (size_t)e],
-                                    &xnod[3 * (size_t)ngeo * e], ngeo, 1.0, coef[e], load, aloc,
-                                    ldk, bloc);
+                st = elem_hcurl_core(no, &nori_e[12 * (size_t)e], &nori_f[6 * (size_t)e],
+                                     &xnod[3 * (size_t)ngeo * e], ngeo, 1.0, coef[e], load, aloc,
+                                     ldk, bloc, blas);
             }
             size_t o2 = (size_t)e * ldk * ldk, o1 = (size_t)e * ldk;
             if (kout)
@@ -1105,12 +1210,10 @@ int orc_element_loop(int space, int nel, const int *norder, const int *nori_e, c
             if (bout) memcpy(&bout[o1], bloc, sizeof(double) * n);
             int ni = n - nb;
             if (st >= 0 && aii) {
-                int s2 = herm ? orc_stc_fwd_herm(ni, nb, aloc, ldk, bloc,
-                                                 aschur ? &aschur[o2] : NULL,
-                                                 bschur ? &bschur[o1] : NULL)
-                              : orc_stc_fwd_gen(ni, nb, aloc, ldk, bloc,
-                                                aschur ? &aschur[o2] : NULL,
-                                                bschur ? &bschur[o1] : NULL);
+                double *as = aschur ? &aschur[o2] : NULL, *bs = bschur ? &bschur[o1] : NULL;
+                int s2 = blas ? stc_fwd_gen_blas(ni, nb, aloc, ldk, bloc, as, bs)
+                         : herm ? orc_stc_fwd_herm(ni, nb, aloc, ldk, bloc, as, bs)
+                                : orc_stc_fwd_gen(ni, nb, aloc, ldk, bloc, as, bs);
                 if (s2) st = s2;
                 for (int j = 0; j < ni; j++)
                     memcpy(&aii[o2 + (size_t)j * ldk], &aloc[(size_t)j * ldk], sizeof(double) * ni);
@@ -1125,6 +1228,28 @@ int orc_element_loop(int space, int nel, const int *norder, const int *nori_e, c
         free(bloc);
     }
     return status;
+}
+
+int orc_element_loop(int space, int nel, const int *norder, const int *nori_e, const int *nori_f,
+                     const double *xnod, int ngeo, const double *coef, int load, int ldk,
+                     double *kout, double *bout, double *aii, double *bi, double *aschur,
+                     double *bschur, int herm, int nthreads)
+{
+    return element_loop_core(space, nel, norder, nori_e, nori_f, xnod, ngeo, coef, load, ldk, kout,
+                             bout, aii, bi, aschur, bschur, herm, nthreads, 0);
+}
+
+/* the same loop with the element's SYRK and the condensation's LU, solves and
+ * Schur GEMMs through the host BLAS/LAPACK of orc_set_blas, as the reference
+ * builds it (elem_opt.F90:131-132, stc_fwd_gen stc.F90:443-509) */
+int orc_element_loop_blas(int space, int nel, const int *norder, const int *nori_e,
+                          const int *nori_f, const double *xnod, int ngeo, const double *coef,
+                          int load, int ldk, double *kout, double *bout, double *aii, double *bi,
+                          double *aschur, double *bschur, int nthreads)
+{
+    if (!g_dsyrk || !g_dgemm || !g_dgetrf || !g_dgetrs) return -5;
+    return element_loop_core(space, nel, norder, nori_e, nori_f, xnod, ngeo, coef, load, ldk, kout,
+                             bout, aii, bi, aschur, bschur, 0, nthreads, 1);
 }
 
 /* ------------------------------------------- solution-level element routines */

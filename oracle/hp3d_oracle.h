@@ -119,6 +119,12 @@ int orc_element_loop(int space, int nel, const int *norder, const int *nori_e, c
                      const double *xnod, int ngeo, const double *coef, int load, int ldk,
                      double *kout, double *bout, double *aii, double *bi, double *aschur,
                      double *bschur, int herm, int nthreads);
+/* host BLAS/LAPACK (Fortran ABI, 32-bit integers) for orc_element_loop_blas */
+int orc_set_blas(void *dsyrk, void *dgemm, void *dgetrf, void *dgetrs);
+int orc_element_loop_blas(int space, int nel, const int *norder, const int *nori_e,
+                          const int *nori_f, const double *xnod, int ngeo, const double *coef,
+                          int load, int ldk, double *kout, double *bout, double *aii, double *bi,
+                          double *aschur, double *bschur, int nthreads);
 
 #ifdef __cplusplus
 }

@@ -380,3 +380,30 @@ def test_order_check_and_negative_jacobian():
     xn[2] *= -1  # inverted element
     _k, _b, st = oracle.elem_h1(uniform(1), Z12, Z6, xn)
     assert st == 1
+
+
+@pytest.mark.parametrize("space", [0, 1])
+def test_blas_element_loop_matches_plain_loop(space):
+    """The CPU baseline's reference-shaped loop (DSYRK, DGETRF/DGETRS/DGEMM
+    from the host BLAS) gives the plain loop's element and condensed systems
+    to 1e-12 (mixed anisotropic H1 orders; H(curl) p = 3)."""
+    from hp3d_amd import mesh
+
+    if space == 0:
+        porder = np.random.default_rng(6).integers(1, 6, size=(3 * 2 * 2, 3))
+        m = mesh.structured((3, 2, 2), porder=porder, perturb=0.1, mesh_seed=6, perm_seed=7)
+        load = oracle.LOAD_SINSINSIN
+        ldk = max(oracle.ndof_h1(no) for no in m.norder)
+    else:
+        m = mesh.structured((2, 1, 1), porder=3, perturb=0.1, mesh_seed=6, perm_seed=7,
+                            coef=np.array([1.5, 3.0]))
+        load = oracle.LOAD_CURL_SIN
+        ldk = max(oracle.ndof_hcurl(no) for no in m.norder)
+    args = (space, m.norder, m.nori_e, m.nori_f, m.xnod(), m.coef, load, ldk)
+    a = oracle.element_loop(*args, nthreads=2)
+    b = oracle.element_loop_blas(*args, nthreads=2)
+    assert a["status"] == 0 and b["status"] == 0
+    for key in ("k", "b", "aii", "bi", "aschur", "bschur"):
+        for e in range(m.nel):
+            nrm = np.linalg.norm(a[key][e])
+            assert np.linalg.norm(a[key][e] - b[key][e]) <= 1e-12 * max(nrm, 1e-300), (key, e)
