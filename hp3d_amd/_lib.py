@@ -17,6 +17,7 @@ LOAD_ZERO, LOAD_SINSINSIN, LOAD_CONV, LOAD_ONE, LOAD_CURL_SIN = 0, 1, 2, 3, 10
 LOAD_POLY2 = 4
 FIELD_ERROR, FIELD_ADJOINT, FIELD_L2 = 0, 1, 2
 OPT_PERSISTENT = 1
+SCRATCH_BYTES = 128 << 20   # HP3D_SCRATCH_BYTES
 
 OK = 0
 NEG_JACOBIAN = 1
@@ -45,6 +46,8 @@ SIGNATURES = {
     "hp3d_gpu_stc_workspace": (_sz, [_i, _i, _i]),
     "hp3d_gpu_celem_batch": (_i, [_i, _i, _i, _p, _p, _p, _p, _i, _p, _i, _p, _i, _p, _p, _p,
                                   _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _p]),
+    "hp3d_gpu_celem_batch_noalloc": (_i, [_i, _i, _i, _p, _p, _p, _p, _i, _p, _i, _p, _i, _p,
+                                          _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _p]),
     "hp3d_gpu_stc_bwd_batch": (_i, [_i, _i, _p, _p, _p, _p, _p, _p, _p]),
     "hp3d_gpu_field_batch": (_i, [_i, _i, _i, _p, _p, _p, _p, _i, _p, _p, _p, _p, _i, _p, _p,
                                   _p, _p]),

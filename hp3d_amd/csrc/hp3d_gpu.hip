@@ -2097,6 +2097,40 @@ static int launch_elem(int space, int precision, const ElemArgs &args, int max_n
     return 0;
 }
 
+// ------------------------------------------------ allocation-free scratch
+
+// Device scratch of hp3d_gpu_celem_batch_noalloc: the workspace of its two-pass
+// bins (per-element geometry slabs, then the condensation slabs), one copy per
+// device, held from module load until the process ends.  In host code the
+// array's name is only a stand-in: kernels get the address that
+// hipGetSymbolAddress returns for the current device (scratch_dev).
+constexpr size_t SCRATCH_BYTES = HP3D_SCRATCH_BYTES;
+static __device__ __align__(256) double scratch[SCRATCH_BYTES / sizeof(double)];
+
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch[MAX_DEVICES] = {nullptr};
+
+// the current device's copy of `scratch`, resolved once per device
+static int scratch_dev(double **ptr)
+{
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEVICES) {
+        snprintf(g.err, sizeof(g.err), "device %d: no scratch slot", dev);
+        return HP3D_E_ARG;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *q = nullptr;
+        HIPCHK(hipGetSymbolAddress(&q, HIP_SYMBOL(scratch)));
+        if (!q) return HP3D_E_HIP;
+        g_scratch[dev] = (double *)q;
+    }
+    *ptr = g_scratch[dev];
+    return 0;
+}
+
 static int max_bubbles(int space, int max_ndof)
 {
     // the bubble block is bounded by the interior space of order MAXP but also
@@ -2146,6 +2180,8 @@ int hp3d_gpu_init(int device, int max_ndof)
         return HP3D_E_INIT;
     }
     HIPCHK(hipSetDevice(device));
+    double *scr = nullptr;
+    if (int r = scratch_dev(&scr)) return r;
     if (max_ndof <= 0) max_ndof = MAXN_E;
     g.max_ndof = max_ndof;
     HIPCHK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
@@ -2346,6 +2382,51 @@ int hp3d_gpu_celem_batch(int space, int precision, int nel, const int *norder, c
     return hp3d_gpu_stc_batch(space, precision, nel, norder, off_k, ld_k, kout, off_v, bout, off_c,
                               0, aii, bi, off_s, aschur, bschur, workspace, status, max_ndof,
                               max_nbubble, stream);
+}
+
+int hp3d_gpu_celem_batch_noalloc(int space, int precision, int nel, const int *norder,
+                                 const int *nori_e, const int *nori_f, const double *xnod, int ngeo,
+                                 const double *coef, int load, const long long *off_k, int ld_k,
+                                 void *kout, const long long *off_v, double *bout,
+                                 const long long *off_c, double *aii, double *bi,
+                                 const long long *off_s, double *aschur, double *bschur,
+                                 int *status, int max_ndof, int max_nbubble, void *stream)
+{
+    if (nel <= 0 || max_ndof < 1 || max_ndof > MAXN_E || max_nbubble < 0 || max_nbubble > max_ndof)
+        return hp3d_gpu_celem_batch(space, precision, nel, norder, nori_e, nori_f, xnod, ngeo, coef,
+                                    load, off_k, ld_k, kout, off_v, bout, off_c, aii, bi, off_s,
+                                    aschur, bschur, nullptr, status, max_ndof, max_nbubble, stream);
+    size_t per = hp3d_gpu_celem_workspace(space, precision, 1, max_ndof, max_nbubble);
+    if (per == 0) // fused path: no workspace
+        return hp3d_gpu_celem_batch(space, precision, nel, norder, nori_e, nori_f, xnod, ngeo, coef,
+                                    load, off_k, ld_k, kout, off_v, bout, off_c, aii, bi, off_s,
+                                    aschur, bschur, nullptr, status, max_ndof, max_nbubble, stream);
+    if (per > SCRATCH_BYTES) {
+        snprintf(g.err, sizeof(g.err),
+                 "celem_batch_noalloc: %zu bytes of workspace per element (max_ndof %d, "
+                 "max_nbubble %d) exceed the %zu-byte scratch",
+                 per, max_ndof, max_nbubble, SCRATCH_BYTES);
+        return HP3D_E_ARG;
+    }
+    if (!norder || !nori_e || !nori_f || !xnod || !coef) return HP3D_E_ARG;
+    double *ws = nullptr;
+    if (int r = scratch_dev(&ws)) return r;
+    // elements in chunks whose workspace fits the scratch, one after the other
+    // on `stream`; the offset arrays index the caller's outputs directly
+    const int chunk = (int)std::min<size_t>(SCRATCH_BYTES / per, (size_t)nel);
+    for (int c0 = 0; c0 < nel; c0 += chunk) {
+        const int cn = std::min(chunk, nel - c0);
+        int r = hp3d_gpu_celem_batch(space, precision, cn, norder + 19 * (size_t)c0,
+                                     nori_e + 12 * (size_t)c0, nori_f + 6 * (size_t)c0,
+                                     xnod + 3 * (size_t)ngeo * c0, ngeo, coef + c0, load,
+                                     off_k ? off_k + c0 : nullptr, ld_k, kout,
+                                     off_v ? off_v + c0 : nullptr, bout,
+                                     off_c ? off_c + c0 : nullptr, aii, bi,
+                                     off_s ? off_s + c0 : nullptr, aschur, bschur, ws,
+                                     status ? status + c0 : nullptr, max_ndof, max_nbubble, stream);
+        if (r) return r;
+    }
+    return 0;
 }
 
 int hp3d_gpu_stc_bwd_batch(int space, int nel, const int *norder, const long long *off_s,

@@ -203,6 +203,29 @@ int hp3d_gpu_celem_batch(int space, int precision, int nel, const int *norder, c
 size_t hp3d_gpu_celem_workspace(int space, int precision, int nel, int max_ndof, int max_nbubble);
 
 /*
+ * hp3d_gpu_celem_batch without a workspace argument: allocates nothing (no
+ * hipMalloc, no hipFree, no allocating library call), for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid it).  No counterpart in the reference.  The two-pass
+ * bins use the library's own device scratch of HP3D_SCRATCH_BYTES (a static
+ * device array, one per device, held until the process ends), filled chunk by
+ * chunk of elements on `stream`; an element whose workspace exceeds it
+ * (hp3d_gpu_celem_workspace(space, precision, 1, max_ndof, max_nbubble) >
+ * HP3D_SCRATCH_BYTES) is rejected with HP3D_E_ARG.  The scratch is shared by
+ * every call on a device and keeps the last call's contents: calls that
+ * overlap on one device (other streams, other threads) race.  Everything else
+ * as in hp3d_gpu_celem_batch.
+ */
+#define HP3D_SCRATCH_BYTES ((size_t)128 << 20)
+int hp3d_gpu_celem_batch_noalloc(int space, int precision, int nel, const int *norder,
+                                 const int *nori_e, const int *nori_f, const double *xnod, int ngeo,
+                                 const double *coef, int load, const long long *off_k, int ld_k,
+                                 void *kout, const long long *off_v, double *bout,
+                                 const long long *off_c, double *aii, double *bi,
+                                 const long long *off_s, double *aschur, double *bschur,
+                                 int *status, int max_ndof, int max_nbubble, void *stream);
+
+/*
  * Back substitution for nel elements (stc_bwd_wrapper + stc_bwd + stc_solout,
  * stc.F90:529-679): element e's dof vector sits at x + off_v[e] in element
  * order; its interface part x[0:ni_e] is read and its bubble part
