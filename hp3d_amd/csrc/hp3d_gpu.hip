@@ -10,9 +10,10 @@
 //   elem_single_kernel  one workgroup per element with N <= 128: K kept in
 //                 registers over the quadrature, then (nb <= 32) the bubble
 //                 block condensed in the same launch.
-//   stc_blocked_kernel  one workgroup per element: blocked Cholesky of the
-//                 bubble block with the forward substitution carried along,
-//                 Aii - Y^T Y and Bi - Y^T y_b, X = L^-T Y, on FP64 MFMA.
+//   stc_strip_kernel  one workgroup of 8 waves per element: left-looking
+//                 blocked Cholesky of the bubble block, then [Abi | Bb] in
+//                 register-resident strips through the forward and backward
+//                 sweeps with Aii - Aib X accumulated on the way, FP64 MFMA.
 //   field_kernel  element integrals of finite-element fields (element_error,
 //                 L2 norm and residual, adjoint coefficient gradient).
 //   gather_sum_kernel, csr_spmv_kernel  device assembly and the global operator.
@@ -1285,33 +1286,10 @@ __global__ __launch_bounds__(WG, R == BR ? 2 : 3) void elem_single_persistent(El
 
 // ------------------------------------------------------------ condensation
 
-// ------------------------------------------------------ blocked condensation
-//
-// Static condensation of element systems already in HBM (the algebra of
-// stc_fwd_herm, stc.F90:338-416; Abb is SPD for these forms), one workgroup
-// per element, FP64 arithmetic, FP32 or FP64 K in.  Per element a workspace
-// slab holds L (nbp x nbp, row-major, lower) and Y (nbp x ldy, row-major),
-// nbp = nb rounded up to 16, ldy = ni + 1 rounded up to 16:
-//   A  row r of L and Y is column ni + r of K (K is symmetric: the reads and
-//      the writes are both contiguous); Y's column ni is the bubble load; the
-//      padding is an identity block in L and zeros in Y
-//   B  right-looking blocked Cholesky of L with panels of 16: wave 0 factors
-//      the diagonal block in LDS, the panel below is solved one row per
-//      thread and staged in LDS, L22 -= L21 L21^T on FP64 MFMA
-//   C  Y = L^-1 [Abi | Bb], left-looking: per panel the row block of L is
-//      staged in LDS, each 16 x 16 tile of Y_k gathers L[k, 0:k] Y[0:k] on
-//      MFMA and is solved against L_kk by one lane per column; every panel of
-//      Y is written once (a right-looking update would rewrite Y per panel)
-//   D  Aii = Kii - Y^T Y and Bi = Bi - Y^T y_b: upper tiles of Y^T Y in 4 x 2
-//      blocks on MFMA, operands straight from the slab, both triangles
-//      written coalesced
-//   C' X = L^-T Y by the same left-looking scheme backwards, in place
-//   E  ASchur = X[:, 0:ni] (nb x ni, leading dimension nb), BSchur = X[:, ni]
-constexpr int SB = 16;
 __host__ __device__ inline int rnd16(int x) { return (x + 15) & ~15; }
 
 struct StcArgs {
-    int space, nel, ld_k, ld_c, nbp_max, ldy_max, n_max, nb_max; // n_max, nb_max: the caller's bounds
+    int space, nel, ld_k, ld_c, nbp_max, n_max, nb_max; // nbp_max sizes the slabs; n_max, nb_max: the caller's bounds
     const int *norder; // order codes, or NULL when nsz gives (n, nb) per element
     const int *nsz;
     const long long *off_k, *off_v, *off_c, *off_s;
@@ -1321,20 +1299,56 @@ struct StcArgs {
     int *status;
 };
 
-__host__ __device__ inline size_t stc_slab(int nbp_max, int ldy_max)
+// ------------------------------------------------- strip condensation
+//
+// Static condensation of one element per workgroup of 8 waves, for elements
+// whose K is in HBM (the two-pass bins and the drop-in); all arithmetic FP64,
+// K read as FP32 or FP64.  Replaces stc_fwd_wrapper + stc_fwd_herm
+// (stc.F90:182-416), with its Schur update Aii = Aii - Aib (Abb^-1 Abi)
+// (stc.F90:404-416):
+//   1. Cholesky Abb = L L^T, left-looking by 16-column block panels.  Block
+//      row i of a panel is owned by wave i mod 8, which gathers
+//      A(i,p) - sum_k L(i,k) L(p,k)^T on MFMA; the owner of the diagonal
+//      block factors it in LDS and inverts it; the panel below is
+//      L(i,p) = A~(i,p) L_pp^-T on MFMA.  L goes to the workspace as 16 x 16
+//      column-major tiles (lower triangle), with L_pp^-1 and L_pp^-T.
+//   2. Z = [Abi | Bb] in strips of CW columns held in registers (block row i
+//      of a strip by wave i mod 8).  Forward S = L^-1 Z and backward
+//      X = L^-T S sweeps, right-looking, one block row per step: its owner
+//      applies the inverted diagonal block and broadcasts the solved rows
+//      through LDS, every wave updates its own rows with the streamed L
+//      tiles.  While the backward sweep broadcasts X's rows, the waves
+//      accumulate Aib X on the strip's columns of Aii (upper triangle and the
+//      strip's diagonal block) and, on the load column, Aib BSchur.
+//   3. The strip's X goes to ASchur / BSchur, Aii = Kii - Aib X (both
+//      triangles) and Bi = Bi - Aib BSchur.
+// Each L tile is read once per sweep per strip; nothing else is re-read.
+
+// workspace tiles of one element with up to nbk 16-row bubble blocks: the
+// lower triangle of L (after the factorisation Mb(i,p) = L_ii^-1 L(i,p) in
+// its place), M(i,p) = L(i,p) L_pp^-1, then L_pp^-1 and L_pp^-T
+__host__ __device__ inline int stc_tl(int i, int k) { return i * (i + 1) / 2 + k; }
+__host__ __device__ inline size_t stc_slab(int nbp_max)
 {
-    return (size_t)nbp_max * nbp_max + (size_t)nbp_max * ldy_max;
+    const int nbk = nbp_max / 16;
+    return (size_t)(nbk * (nbk + 1) + 2 * nbk) * 256;
 }
 
-template <typename T, int PLD>
-__global__ __launch_bounds__(WG, 2) void stc_blocked_kernel(StcArgs a)
+template <typename T, int CW, int MAXR, int NW, int RG>
+__global__ __launch_bounds__(64 * NW) void stc_strip_kernel(StcArgs a)
 {
     typedef Mfma<double> MD;
-    __shared__ double l11[SB * 17];
-    __shared__ double P[SB * PLD];          // L21 panel, column k at P + k * PLD
-    __shared__ double stage[4 * 16 * 17];
+    constexpr int WGS = 64 * NW;        // NW waves
+    constexpr int CT = CW / 16;         // tile columns of a strip
+    constexpr int STR = CW + 2;         // LDS row stride of the strip
+    constexpr int NBPX = 16 * NW * MAXR; // largest padded bubble block
+    __shared__ double xs[NBPX * STR];   // the strip: solved block rows as they are broadcast
+    __shared__ double dg[16 * 17], li[16 * 17];
+    __shared__ double stg[NW][16 * 17];
     __shared__ int s_n, s_nb, s_fail;
-    const int e = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int e = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lr = lane & 15, lq = lane >> 4; // A/B operand: row lr, k-slice lq
     if (tid == 0) {
         int n = 0, nb = 0;
         bool ok = true;
@@ -1369,11 +1383,11 @@ __global__ __launch_bounds__(WG, 2) void stc_blocked_kernel(StcArgs a)
     double *aii = a.aii + a.off_c[e];
     double *bi = a.bi + a.off_v[e];
     if (nb == 0) { // nothing to condense: copy through (stc.F90:207)
-        for (int idx = tid; idx < ni * ni; idx += WG) {
+        for (int idx = tid; idx < ni * ni; idx += WGS) {
             int i = idx % ni, j = idx / ni;
             aii[j * ldc + i] = (double)K[j * ldk + i];
         }
-        for (int i = tid; i < ni; i += WG) bi[i] = b[i];
+        for (int i = tid; i < ni; i += WGS) bi[i] = b[i];
         if (tid == 0 && a.status) a.status[e] = a.status[e] == HP3D_NEG_JACOBIAN ? HP3D_NEG_JACOBIAN : 0;
         return;
     }
@@ -1381,271 +1395,457 @@ __global__ __launch_bounds__(WG, 2) void stc_blocked_kernel(StcArgs a)
         if (tid == 0 && a.status) a.status[e] = HP3D_E_NOIFACE;
         return;
     }
-    const int nbp = rnd16(nb), ldy = rnd16(ni + 1);
-    if (nbp > a.nbp_max || ldy > a.ldy_max || nbp - SB > PLD) { // slab or panel too small
+    const int nbp = rnd16(nb), nbk = nbp / 16;
+    const int nib = (ni + 15) / 16; // block rows of Aii
+    if (nbp > a.nbp_max || nbk > NW * MAXR) { // slab or strip too small
         if (tid == 0 && a.status) a.status[e] = HP3D_E_ARG;
         return;
     }
-    double *L = a.ws + (size_t)e * stc_slab(a.nbp_max, a.ldy_max);
-    double *Y = L + (size_t)nbp * nbp;
+    const int nt = nbk * (nbk + 1) / 2;
+    double *W = a.ws + (size_t)e * stc_slab(a.nbp_max);
+    double *Mt = W + (size_t)nt * 256;
+    double *Linv = Mt + (size_t)nt * 256, *LinvT = Linv + (size_t)nbk * 256;
+    double *st = stg[wave];
+    const int ab = lr + 16 * lq;    // a tile's A(row lr, col lq) in column-major, k-step 0
+    const int abT = lq + 16 * lr;   // the transposed tile's
+    // Loads are unconditional, from an index clamped to 0 where the value is
+    // not wanted, and their values are converted or masked only where they
+    // are used: a load whose value is used inside a lane-predicated block is
+    // waited for right there, which serialises the loads.
 
-    // A: gather
-    for (int r = wave; r < nbp; r += WG / 64) {
-        const T *kc = K + (size_t)(ni + r) * ldk;
-        // lower triangle only: no phase reads L above the diagonal as data
-        for (int c = lane; c <= r; c += 64)
-            L[(size_t)r * nbp + c] = r < nb ? (c < nb ? (double)kc[ni + c] : 0.0) : (c == r ? 1.0 : 0.0);
-        for (int c = lane; c < ldy; c += 64) {
-            double v = 0.0;
-            if (r < nb) v = c < ni ? (double)kc[c] : (c == ni ? b[ni + r] : 0.0);
-            Y[(size_t)r * ldy + c] = v;
+    // ---- 1: Cholesky, left-looking by block panels
+    for (int p = 0; p < nbk; ++p) {
+        // A(i,p) of Abb (K symmetric: read along rows), padded with the identity
+        T araw[MAXR][4];
+#pragma unroll
+        for (int r = 0; r < MAXR; ++r) {
+            const int i = wave + NW * r;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int rr = 16 * i + MD::row(lane, j), cc = 16 * p + lr;
+                const bool v = i >= p && i < nbk && rr < nb && cc < nb;
+                araw[r][j] = K[v ? (size_t)(ni + cc) + (size_t)(ni + rr) * ldk : 0];
+            }
         }
-    }
-    __syncthreads();
-
-    // B: right-looking blocked Cholesky of Abb (L only)
-    const int np = nbp / SB;
-    const int nyc = ldy / SB;
-    for (int p = 0; p < np; ++p) {
-        const int k0 = SB * p;
-        // diagonal block: factored by wave 0 (wave-level ordering only)
-        if (wave == 0) {
-            for (int idx = lane; idx < SB * SB; idx += 64)
-                l11[(idx >> 4) * 17 + (idx & 15)] = L[(size_t)(k0 + (idx >> 4)) * nbp + k0 + (idx & 15)];
-            wave_sync_lds();
-            for (int j = 0; j < SB; ++j) {
+        // sum_k L(i,k) L(p,k)^T, the tiles of step k + 1 loaded during step k
+        d4_t csum[MAXR];
+#pragma unroll
+        for (int r = 0; r < MAXR; ++r) csum[r] = d4_t{0.0, 0.0, 0.0, 0.0};
+        double ta[MAXR][4], tb[4];
+        auto ld_k = [&](int k) {
+            const double *Lpk = W + (size_t)stc_tl(p, k) * 256;
+#pragma unroll
+            for (int s = 0; s < 4; ++s) tb[s] = Lpk[ab + 64 * s];
+#pragma unroll
+            for (int r = 0; r < MAXR; ++r) {
+                const int i = wave + NW * r;
+                const double *Lik = (i >= p && i < nbk) ? W + (size_t)stc_tl(i, k) * 256 : Lpk;
+#pragma unroll
+                for (int s = 0; s < 4; ++s) ta[r][s] = Lik[ab + 64 * s];
+            }
+        };
+        ld_k(0); // (p = 0: the diagonal tile, unused)
+        for (int k = 0; k < p; ++k) {
+            double ca[MAXR][4], cb[4];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                cb[s] = tb[s];
+#pragma unroll
+                for (int r = 0; r < MAXR; ++r) ca[r][s] = ta[r][s];
+            }
+            ld_k(min(k + 1, p - 1));
+#pragma unroll
+            for (int r = 0; r < MAXR; ++r) {
+                const int i = wave + NW * r;
+                if (i < p || i >= nbk) continue;
+#pragma unroll
+                for (int s = 0; s < 4; ++s) csum[r] = MD::mma(ca[r][s], cb[s], csum[r]);
+            }
+        }
+        d4_t cacc[MAXR];
+#pragma unroll
+        for (int r = 0; r < MAXR; ++r) {
+            const int i = wave + NW * r;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int rr = 16 * i + MD::row(lane, j), cc = 16 * p + lr;
+                const double av = (rr < nb && cc < nb) ? (double)araw[r][j] : (rr == cc ? 1.0 : 0.0);
+                cacc[r][j] = av - csum[r][j];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < MAXR; ++r)
+            if (wave + NW * r == p) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) dg[MD::row(lane, j) * 17 + lr] = cacc[r][j];
+            }
+        __syncthreads();
+        if (wave == p % NW) {
+            // factor the diagonal block in LDS (lower triangle)
+            for (int jp = 0; jp < 16; ++jp) {
                 if (lane == 0) {
-                    double d = l11[j * 17 + j];
+                    double d = dg[jp * 17 + jp];
                     if (!(d > 0.0)) {
                         s_fail = 1;
                         d = 1.0;
                     }
-                    l11[j * 17 + j] = sqrt(d);
+                    dg[jp * 17 + jp] = sqrt(d);
                 }
                 wave_sync_lds();
-                if (lane > j && lane < SB) l11[lane * 17 + j] /= l11[j * 17 + j];
+                if (lane > jp && lane < 16) dg[lane * 17 + jp] /= dg[jp * 17 + jp];
                 wave_sync_lds();
-                for (int idx = lane; idx < SB * SB; idx += 64) {
-                    int i = idx >> 4, m = idx & 15;
-                    if (i > j && m > j && m <= i) l11[i * 17 + m] -= l11[i * 17 + j] * l11[m * 17 + j];
+                for (int idx = lane; idx < 256; idx += 64) {
+                    const int i = idx >> 4, m = idx & 15;
+                    if (i > jp && m > jp && m <= i) dg[i * 17 + m] -= dg[i * 17 + jp] * dg[m * 17 + jp];
                 }
                 wave_sync_lds();
             }
-            for (int idx = lane; idx < SB * SB; idx += 64) {
-                int i = idx >> 4, j = idx & 15;
-                if (j <= i) L[(size_t)(k0 + i) * nbp + k0 + j] = l11[i * 17 + j];
+            // L_pp^-1, one column per lane
+            if (lane < 16) {
+                double x[16];
+#pragma unroll
+                for (int m = 0; m < 16; ++m) {
+                    // keeps row m's L reads inside the iteration (hoisted,
+                    // the 136 reads would take 272 VGPRs)
+                    __asm__ volatile("" ::: "memory");
+                    double s = m == lane ? 1.0 : 0.0;
+#pragma unroll
+                    for (int k = 0; k < m; ++k) s -= dg[m * 17 + k] * x[k];
+                    x[m] = m < lane ? 0.0 : s / dg[m * 17 + m];
+                }
+#pragma unroll
+                for (int m = 0; m < 16; ++m) li[m * 17 + lane] = x[m];
+            }
+            wave_sync_lds();
+            double *Lpp = W + (size_t)stc_tl(p, p) * 256;
+            for (int idx = lane; idx < 256; idx += 64) {
+                const int r = idx & 15, c = idx >> 4; // column-major tiles
+                Lpp[idx] = r >= c ? dg[r * 17 + c] : 0.0;
+                Linv[(size_t)p * 256 + idx] = li[r * 17 + c];
+                LinvT[(size_t)p * 256 + idx] = li[c * 17 + r];
             }
         }
         __syncthreads();
-        const int nr = nbp - k0 - SB;
-        for (int t = tid; t < nr; t += WG) {
-            // keeps the 136 L11 reads inside the loop (hoisted, they cost
-            // 272 VGPRs and the kernel's occupancy)
-            __asm__ volatile("" ::: "memory");
-            const size_t row = (size_t)(k0 + SB + t) * nbp + k0;
-            double x[SB];
+        // the panel below: L(i,p) = A~(i,p) L_pp^-T
 #pragma unroll
-            for (int j = 0; j < SB; ++j) x[j] = L[row + j];
+        for (int r = 0; r < MAXR; ++r) {
+            const int i = wave + NW * r;
+            if (i <= p || i >= nbk) continue;
 #pragma unroll
-            for (int j = 0; j < SB; ++j) {
-                double sx = x[j];
+            for (int j = 0; j < 4; ++j) st[MD::row(lane, j) * 17 + lr] = cacc[r][j];
+            wave_sync_lds();
+            d4_t d = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-                for (int m = 0; m < j; ++m) sx -= x[m] * l11[j * 17 + m];
-                x[j] = sx / l11[j * 17 + j];
+            for (int s = 0; s < 4; ++s) {
+                const int kk = 4 * s + lq;
+                d = MD::mma(st[lr * 17 + kk], li[lr * 17 + kk], d);
             }
+            wave_sync_lds();
+            double *Lip = W + (size_t)stc_tl(i, p) * 256;
 #pragma unroll
-            for (int j = 0; j < SB; ++j) {
-                L[row + j] = x[j];
-                P[j * PLD + t] = x[j];
-            }
-        }
-        __syncthreads();
-        const int nt = nr / SB, nL = nt * (nt + 1) / 2;
-        for (int t = wave; t < nL; t += WG / 64) {
-            int I, J;
-            block_ij(t, J, I); // I >= J: lower tiles of the trailing block
-            d4_t acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-            for (int ks = 0; ks < SB; ks += 4) {
-                const int kk = ks + (lane >> 4);
-                acc = MD::mma(P[kk * PLD + SB * I + (lane & 15)], P[kk * PLD + SB * J + (lane & 15)], acc);
-            }
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-                L[(size_t)(k0 + SB + SB * I + MD::row(lane, j)) * nbp + k0 + SB + SB * J + (lane & 15)] -= acc[j];
+            for (int j = 0; j < 4; ++j) Lip[MD::row(lane, j) + 16 * lr] = d[j];
         }
         __syncthreads();
     }
 
-    double *stw = stage + wave * 16 * 17;
-    // Y rows r0..r0+15 -= acc (a tile in the MFMA layout, columns c0..)
-    auto tile_sub = [&](const d4_t &acc, int r0, int c0) {
+    // ---- the sweeps' tiles: -M(i,p) = -L(i,p) L_pp^-1 (forward) and
+    // -Mb(i,p) = -L_ii^-1 L(i,p) in place of L(i,p) (backward), i > p, so that
+    // a step broadcasts its block row before the diagonal solve (the row's
+    // owner solves it off the critical path)
+    for (int tix = wave; tix < nt; tix += NW) {
+        int i = 0;
+        while ((i + 1) * (i + 2) / 2 <= tix) ++i;
+        const int pcol = tix - i * (i + 1) / 2;
+        if (pcol == i) continue;
+        double *Lt = W + (size_t)tix * 256;
+        const double *Dp = Linv + (size_t)pcol * 256, *Di = Linv + (size_t)i * 256;
+        d4_t m = {0.0, 0.0, 0.0, 0.0}, mb = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int j = 0; j < 4; ++j) Y[(size_t)(r0 + MD::row(lane, j)) * ldy + c0 + (lane & 15)] -= acc[j];
-    };
-    // rows k0..k0+15 of every column of Y: solve with L_kk (forward) or
-    // L_kk^T (backward), one column per thread
-    auto panel_solve = [&](int k0, bool backward) {
-        for (int c = tid; c < ldy; c += WG) {
-            __asm__ volatile("" ::: "memory");
-            double y[SB];
-#pragma unroll
-            for (int j = 0; j < SB; ++j) y[j] = Y[(size_t)(k0 + j) * ldy + c];
-            if (!backward) {
-#pragma unroll
-                for (int j = 0; j < SB; ++j) {
-                    double sy = y[j];
-#pragma unroll
-                    for (int m = 0; m < j; ++m) sy -= l11[j * 17 + m] * y[m];
-                    y[j] = sy / l11[j * 17 + j];
-                }
-            } else {
-#pragma unroll
-                for (int j = SB - 1; j >= 0; --j) {
-                    double sy = y[j];
-#pragma unroll
-                    for (int m = j + 1; m < SB; ++m) sy -= l11[m * 17 + j] * y[m];
-                    y[j] = sy / l11[j * 17 + j];
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < SB; ++j) Y[(size_t)(k0 + j) * ldy + c] = y[j];
+        for (int s2 = 0; s2 < 4; ++s2) {
+            m = MD::mma(Lt[ab + 64 * s2], Dp[abT + 4 * s2], m);
+            mb = MD::mma(Di[ab + 64 * s2], Lt[abT + 4 * s2], mb);
         }
-    };
-
-    // C: forward substitution Y = L^-1 [Abi | Bb], left-looking: panel k is
-    // Y_k = L_kk^-1 (C_k - L[k, 0:k] Y[0:k]), written once
-    for (int p = 0; p < np; ++p) {
-        const int k0 = SB * p;
-        // L row panel (16 x k0) as MFMA A operand, column kk at P + 16 kk; and L_kk
-        for (int idx = tid; idx < SB * k0; idx += WG) {
-            const int r = idx / k0, c = idx % k0;
-            P[c * SB + r] = L[(size_t)(k0 + r) * nbp + c];
-        }
-        {
-            int i = tid >> 4, j = tid & 15;
-            l11[i * 17 + j] = L[(size_t)(k0 + i) * nbp + k0 + j];
-        }
-        __syncthreads();
-        if (k0 > 0) {
-            for (int J = wave; J < nyc; J += WG / 64) {
-                d4_t acc = {0.0, 0.0, 0.0, 0.0};
-                for (int kk0 = 0; kk0 < k0; kk0 += 4) {
-                    const int kk = kk0 + (lane >> 4);
-                    acc = MD::mma(P[kk * SB + (lane & 15)], Y[(size_t)kk * ldy + SB * J + (lane & 15)], acc);
-                }
-                tile_sub(acc, k0, SB * J);
-            }
-            __syncthreads();
-        }
-        panel_solve(k0, false);
-        __syncthreads();
-    }
-
-    // D: Aii = Kii - Y^T Y, Bi = Bi - Y^T y_b: upper tiles in 4 x 2 blocks
-    // (eight MFMAs per six operand loads), operands straight from the slab
-    {
-        constexpr int BRI = 4, BRJ = 2;
-        const int nbi = (nyc + BRI - 1) / BRI, nbj = (nyc + BRJ - 1) / BRJ;
-        // block (BI, BJ) covers tile rows BRI*BI.., columns BRJ*BJ..; keep the
-        // blocks that hold an upper tile
-        for (int t = wave; t < nbi * nbj; t += WG / 64) {
-            const int BI = t / nbj, BJ = t % nbj;
-            if (BRI * BI > BRJ * BJ + BRJ - 1) continue;
-            d4_t acc[BRI][BRJ];
 #pragma unroll
-            for (int a0 = 0; a0 < BRI; ++a0)
-#pragma unroll
-                for (int b0 = 0; b0 < BRJ; ++b0) acc[a0][b0] = d4_t{0.0, 0.0, 0.0, 0.0};
-            for (int k = 0; k < nbp; k += 4) {
-                const double *yr = Y + (size_t)(k + (lane >> 4)) * ldy + (lane & 15);
-                double av[BRI], bv[BRJ];
-#pragma unroll
-                for (int a0 = 0; a0 < BRI; ++a0) av[a0] = BRI * BI + a0 < nyc ? yr[SB * (BRI * BI + a0)] : 0.0;
-#pragma unroll
-                for (int b0 = 0; b0 < BRJ; ++b0) bv[b0] = BRJ * BJ + b0 < nyc ? yr[SB * (BRJ * BJ + b0)] : 0.0;
-#pragma unroll
-                for (int a0 = 0; a0 < BRI; ++a0)
-#pragma unroll
-                    for (int b0 = 0; b0 < BRJ; ++b0) acc[a0][b0] = MD::mma(av[a0], bv[b0], acc[a0][b0]);
-            }
-#pragma unroll
-            for (int a0 = 0; a0 < BRI; ++a0)
-#pragma unroll
-                for (int b0 = 0; b0 < BRJ; ++b0) {
-                    const int I = BRI * BI + a0, J = BRJ * BJ + b0;
-                    if (I >= nyc || J >= nyc || I > J) continue;
-                    d4_t v = acc[a0][b0];
-                    const int cj = SB * J + (lane & 15);
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        const int ri = SB * I + MD::row(lane, j);
-                        if (cj == ni && ri < ni) bi[ri] = b[ri] - v[j];
-                        double kv = (ri < ni && cj < ni) ? (double)K[(size_t)ri * ldk + cj] : 0.0;
-                        v[j] = kv - v[j];
-                    }
-                    write_sym_tile<d4_t, double, MD>(v, SB * I, SB * J, ni, aii, ldc, stw, lane);
-                }
+        for (int j = 0; j < 4; ++j) { // stored negated: the sweeps subtract
+            Mt[(size_t)tix * 256 + MD::row(lane, j) + 16 * lr] = -m[j];
+            Lt[MD::row(lane, j) + 16 * lr] = -mb[j];
         }
     }
     __syncthreads();
 
-    // back substitution X = L^-T Y, left-looking: panel k is
-    // X_k = L_kk^-T (Y_k - L[k+1:, k]^T X[k+1:]), in place over Y
-    for (int p = np - 1; p >= 0; --p) {
-        const int k0 = SB * p, rb = nbp - k0 - SB;
-        // L column panel below the diagonal block as MFMA A operand
-        // (A[i][kk] = L(k0 + 16 + kk, k0 + i)), and L_kk
-        for (int idx = tid; idx < SB * rb; idx += WG) {
-            const int kk = idx / SB, i = idx % SB;
-            P[kk * SB + i] = L[(size_t)(k0 + SB + kk) * nbp + k0 + i];
-        }
+    // ---- 2: strips of [Abi | Bb]
+    const int ncol = ni + 1;
+    for (int c0 = 0; c0 < ncol; c0 += CW) {
+        d4_t acc[MAXR][CT];
+        // Z(r, c) = Abi(r, c) (K symmetric: read along rows of Aib), Bb in column ni
         {
-            int i = tid >> 4, j = tid & 15;
-            l11[i * 17 + j] = L[(size_t)(k0 + i) * nbp + k0 + j];
-        }
-        __syncthreads();
-        if (rb > 0) {
-            for (int J = wave; J < nyc; J += WG / 64) {
-                d4_t acc = {0.0, 0.0, 0.0, 0.0};
-                for (int kk0 = 0; kk0 < rb; kk0 += 4) {
-                    const int kk = kk0 + (lane >> 4);
-                    acc = MD::mma(P[kk * SB + (lane & 15)], Y[(size_t)(k0 + SB + kk) * ldy + SB * J + (lane & 15)],
-                                  acc);
+            T zraw[MAXR][CT][4];
+            double braw[MAXR][4];
+#pragma unroll
+            for (int r = 0; r < MAXR; ++r) {
+                const int i = wave + NW * r;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int rb = 16 * i + MD::row(lane, j);
+                    const bool vr = i < nbk && rb < nb;
+                    braw[r][j] = b[vr ? ni + rb : 0];
+#pragma unroll
+                    for (int t = 0; t < CT; ++t) {
+                        const int c = c0 + 16 * t + lr;
+                        zraw[r][t][j] = K[(vr && c < ni) ? (size_t)c + (size_t)(ni + rb) * ldk : 0];
+                    }
                 }
-                tile_sub(acc, k0, SB * J);
             }
-            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < MAXR; ++r) {
+                const int i = wave + NW * r;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int rb = 16 * i + MD::row(lane, j);
+                    const bool vr = i < nbk && rb < nb;
+#pragma unroll
+                    for (int t = 0; t < CT; ++t) {
+                        const int c = c0 + 16 * t + lr;
+                        acc[r][t][j] = !vr ? 0.0 : (c < ni ? (double)zraw[r][t][j] : (c == ni ? braw[r][j] : 0.0));
+                    }
+                }
+            }
         }
-        panel_solve(k0, true);
-        __syncthreads();
+        // the owner's updated block row p goes to rows 16p.. of the strip in
+        // LDS, then is solved in registers against a diagonal tile (column-
+        // major D): acc <- D acc
+        auto put_row = [&](const d4_t (&row)[CT], int p) {
+            double *buf = xs + (size_t)16 * p * STR;
+#pragma unroll
+            for (int t = 0; t < CT; ++t)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) buf[MD::row(lane, j) * STR + 16 * t + lr] = row[t][j];
+        };
+        auto solve_row = [&](d4_t (&row)[CT], const double (&dA)[4], int p) {
+            const double *buf = xs + (size_t)16 * p * STR;
+            wave_sync_lds();
+#pragma unroll
+            for (int t = 0; t < CT; ++t) {
+                d4_t d = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int s2 = 0; s2 < 4; ++s2) d = MD::mma(dA[s2], buf[(4 * s2 + lq) * STR + 16 * t + lr], d);
+                row[t] = d;
+            }
+        };
+        // one sweep: step p broadcasts the owner's updated row p, every wave
+        // subtracts T(i,p) row_p from its rows i (forward: i > p, T = M;
+        // backward: i < p, T = Mb(p,i)^T) and the owner of the next row
+        // updates and broadcasts it first; the tiles of step p + 1 are loaded
+        // during step p
+        auto sweep = [&](bool fwd) {
+            double nL[MAXR][4];
+            auto ld_step = [&](int p) {
+#pragma unroll
+                for (int r = 0; r < MAXR; ++r) {
+                    const int i = wave + NW * r;
+                    const bool v = fwd ? (i > p && i < nbk) : (i < p);
+                    const double *Tt = !v ? W : fwd ? Mt + (size_t)stc_tl(i, p) * 256 : W + (size_t)stc_tl(p, i) * 256;
+#pragma unroll
+                    for (int s2 = 0; s2 < 4; ++s2) nL[r][s2] = Tt[(fwd ? ab + 64 * s2 : abT + 4 * s2)];
+                }
+            };
+            const int p0 = fwd ? 0 : nbk - 1, dp = fwd ? 1 : -1;
+            const double *Dg = fwd ? Linv : LinvT;
+            double dA[4]; // the diagonal tile of the coming step's row
+            ld_step(p0);
+#pragma unroll
+            for (int s2 = 0; s2 < 4; ++s2) dA[s2] = Dg[(size_t)p0 * 256 + ab + 64 * s2];
+            if (wave == p0 % NW) {
+#pragma unroll
+                for (int r = 0; r < MAXR; ++r)
+                    if (r == p0 / NW) {
+                        put_row(acc[r], p0);
+                        solve_row(acc[r], dA, p0);
+                    }
+            }
+            for (int p = p0; p >= 0 && p < nbk; p += dp) {
+                double cL[MAXR][4];
+#pragma unroll
+                for (int r = 0; r < MAXR; ++r)
+#pragma unroll
+                    for (int s2 = 0; s2 < 4; ++s2) cL[r][s2] = nL[r][s2];
+                const int pn = p + dp;
+                const bool more = pn >= 0 && pn < nbk;
+                const int pc = more ? pn : p; // loads stay unconditional
+                ld_step(pc);
+#pragma unroll
+                for (int s2 = 0; s2 < 4; ++s2) dA[s2] = Dg[(size_t)pc * 256 + ab + 64 * s2];
+                const bool own_next = more && wave == pn % NW;
+                lds_barrier();
+                const double *buf = xs + (size_t)16 * p * STR;
+                auto upd = [&](int r) {
+#pragma unroll
+                    for (int s2 = 0; s2 < 4; ++s2) {
+#pragma unroll
+                        for (int t = 0; t < CT; ++t)
+                            acc[r][t] = MD::mma(cL[r][s2], buf[(4 * s2 + lq) * STR + 16 * t + lr], acc[r][t]);
+                    }
+                };
+                // the next row first: update, broadcast, solve
+#pragma unroll
+                for (int r = 0; r < MAXR; ++r)
+                    if (own_next && r == pn / NW) {
+                        upd(r);
+                        put_row(acc[r], pn);
+                        solve_row(acc[r], dA, pn);
+                    }
+#pragma unroll
+                for (int r = 0; r < MAXR; ++r) {
+                    const int i = wave + NW * r;
+                    if (own_next && r == pn / NW) continue;
+                    if (fwd ? (i <= p || i >= nbk) : (i >= p)) continue;
+                    upd(r);
+                }
+            }
+        };
+        sweep(true);   // S = L^-1 Z (rows of S in registers)
+        lds_barrier(); // every wave has read the forward rows
+        sweep(false);  // X = L^-T S
+        lds_barrier(); // every wave has read the backward rows
+        // the solved X into the strip in LDS (for Aib X)
+#pragma unroll
+        for (int r = 0; r < MAXR; ++r) {
+            const int i = wave + NW * r;
+            if (i < nbk) put_row(acc[r], i);
+        }
+        // outputs: X -> ASchur (column-major, leading dimension nb), BSchur
+        double *X = a.aschur + a.off_s[e];
+        double *BS = a.bschur + a.off_v[e];
+#pragma unroll
+        for (int r = 0; r < MAXR; ++r) {
+            const int i = wave + NW * r;
+            if (i >= nbk) continue;
+#pragma unroll
+            for (int t = 0; t < CT; ++t) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) st[MD::row(lane, j) * 17 + lr] = acc[r][t][j];
+                wave_sync_lds();
+                const int rb = 16 * i + lr;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int c = c0 + 16 * t + lq + 4 * j; // column of element (lr, lq + 4j)
+                    const double v = st[lr * 17 + lq + 4 * j];
+                    if (rb < nb) {
+                        if (c < ni) X[(size_t)c * nb + rb] = v;
+                        else if (c == ni) BS[rb] = v;
+                    }
+                }
+                wave_sync_lds();
+            }
+        }
+        lds_barrier(); // the whole strip X in LDS
+        // Aib X on this strip's columns: Aii rows up to the strip's last
+        // column (upper triangle and the strip's diagonal block), all rows on
+        // the load column's strip.  Block row I of Aii goes to wave I mod 8,
+        // RG rows at a time with all CT tile columns, so each Aib value feeds
+        // CT MFMAs.
+        const int nI = min(nib, (c0 + CW) / 16);
+        for (int I0 = wave; I0 < nI; I0 += NW * RG) {
+            d4_t accA[RG][CT];
+#pragma unroll
+            for (int g = 0; g < RG; ++g)
+#pragma unroll
+                for (int t = 0; t < CT; ++t) accA[g][t] = d4_t{0.0, 0.0, 0.0, 0.0};
+            // Aib(ri, kb) = K(ri, ni + kb), the values of step p + 1 loaded
+            // during step p
+            T nA[RG][4];
+            auto ld_A = [&](int p) {
+#pragma unroll
+                for (int g = 0; g < RG; ++g) {
+                    const int ri = 16 * (I0 + NW * g) + lr;
+#pragma unroll
+                    for (int s2 = 0; s2 < 4; ++s2) {
+                        const int kb = 16 * p + 4 * s2 + lq;
+                        nA[g][s2] = K[(ri < ni && kb < nb) ? (size_t)ri + (size_t)(ni + kb) * ldk : 0];
+                    }
+                }
+            };
+            ld_A(0);
+            for (int p = 0; p < nbk; ++p) {
+                double cA[RG][4];
+#pragma unroll
+                for (int g = 0; g < RG; ++g) {
+                    const int ri = 16 * (I0 + NW * g) + lr;
+#pragma unroll
+                    for (int s2 = 0; s2 < 4; ++s2) {
+                        const int kb = 16 * p + 4 * s2 + lq;
+                        cA[g][s2] = (ri < ni && kb < nb) ? (double)nA[g][s2] : 0.0;
+                    }
+                }
+                ld_A(min(p + 1, nbk - 1));
+#pragma unroll
+                for (int s2 = 0; s2 < 4; ++s2) {
+#pragma unroll
+                    for (int t = 0; t < CT; ++t) {
+                        const double bv = xs[(size_t)(16 * p + 4 * s2 + lq) * STR + 16 * t + lr];
+#pragma unroll
+                        for (int g = 0; g < RG; ++g) accA[g][t] = MD::mma(cA[g][s2], bv, accA[g][t]);
+                    }
+                }
+            }
+            // Aii = Kii - Aib X (both triangles), Bi = Bi - Aib BSchur
+#pragma unroll
+            for (int g = 0; g < RG; ++g) {
+                const int I = I0 + NW * g;
+                if (I >= nI) continue;
+#pragma unroll
+                for (int t = 0; t < CT; ++t) {
+                    d4_t v4 = accA[g][t];
+                    T kraw[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const int r = 16 * I + MD::row(lane, j), c = c0 + 16 * t + lr;
+                        kraw[j] = K[(r < ni && c < ni) ? (size_t)c + (size_t)r * ldk : 0];
+                    }
+                    // mirrored entries (c, r) first, straight from the
+                    // accumulator: lanes run along c, i.e. along the rows of column r
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const int r = 16 * I + MD::row(lane, j), c = c0 + 16 * t + lr;
+                        if (r >= ni) continue;
+                        if (c < ni) {
+                            const double v = (double)kraw[j] - v4[j];
+                            v4[j] = v;
+                            if (r < c0) aii[(size_t)c + (size_t)r * ldc] = v; // (c, r) of the lower part
+                        } else if (c == ni) {
+                            bi[r] = b[r] - v4[j];
+                        }
+                    }
+                    // the entries (r, c) themselves, transposed through LDS so lanes run along r
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) st[MD::row(lane, j) * 17 + lr] = v4[j];
+                    wave_sync_lds();
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const int r = 16 * I + lr, c = c0 + 16 * t + lq + 4 * j;
+                        if (r < ni && c < ni) aii[(size_t)r + (size_t)c * ldc] = st[lr * 17 + lq + 4 * j];
+                    }
+                    wave_sync_lds();
+                }
+            }
+        }
+        lds_barrier(); // the strip's LDS is read before the next strip writes it
     }
-
-    // E: outputs
-    double *X = a.aschur + a.off_s[e];
-    for (int idx = tid; idx < nb * ni; idx += WG) {
-        const int r = idx % nb, c = idx / nb;
-        X[(size_t)c * nb + r] = Y[(size_t)r * ldy + c];
-    }
-    for (int r = tid; r < nb; r += WG) a.bschur[a.off_v[e] + r] = Y[(size_t)r * ldy + ni];
+    __syncthreads();
     // the element pass's negative-Jacobian warning survives a successful factorisation
     if (tid == 0 && a.status)
         a.status[e] = s_fail ? HP3D_E_FACTOR : (a.status[e] == HP3D_NEG_JACOBIAN ? HP3D_NEG_JACOBIAN : 0);
 }
 
-// panel buffer lengths (in doubles) of the two instantiations: nb up to 144
-// (all H1 elements, H(curl) to p = 4) and up to 464 (H(curl) p = 6); each
-// length is 16 mod 32 doubles so that the 16 rows an MFMA operand read
-// touches fall in distinct LDS banks
-constexpr int STC_PLD_S = 144, STC_PLD_L = 464;
-
 template <typename T>
 static void launch_stc(StcArgs a, int max_nbubble, hipStream_t st)
 {
-    if (rnd16(max_nbubble) - SB <= STC_PLD_S)
-        hipLaunchKernelGGL((stc_blocked_kernel<T, STC_PLD_S>), dim3(a.nel), dim3(WG), 0, st, a);
+    // 64-column strips (2 block rows per wave) up to 256 bubbles, 32-column
+    // strips (4 block rows per wave) beyond
+    if (rnd16(max_nbubble) <= 256)
+        hipLaunchKernelGGL((stc_strip_kernel<T, 64, 2, 8, 2>), dim3(a.nel), dim3(512), 0, st, a);
     else
-        hipLaunchKernelGGL((stc_blocked_kernel<T, STC_PLD_L>), dim3(a.nel), dim3(WG), 0, st, a);
+        hipLaunchKernelGGL((stc_strip_kernel<T, 32, 4, 8, 2>), dim3(a.nel), dim3(512), 0, st, a);
 }
 
 struct BwdArgs {
@@ -2196,7 +2396,7 @@ int hp3d_gpu_init(int device, int max_ndof)
     HIPCHK(hipMalloc(&g.d_as, nn * sizeof(double)));
     HIPCHK(hipMalloc(&g.d_bs, max_ndof * sizeof(double)));
     g.nbm = max_bubbles(HP3D_HCURL, max_ndof);
-    HIPCHK(hipMalloc(&g.d_ws, std::max(stc_slab(rnd16(g.nbm), rnd16(max_ndof + 1)), geo_slab(HP3D_HCURL)) *
+    HIPCHK(hipMalloc(&g.d_ws, std::max(stc_slab(rnd16(g.nbm)), geo_slab(HP3D_HCURL)) *
                                   sizeof(double)));
     HIPCHK(hipMalloc(&g.d_off, sizeof(long long)));
     HIPCHK(hipMemset(g.d_off, 0, sizeof(long long)));
@@ -2274,7 +2474,7 @@ size_t hp3d_gpu_elem_workspace(int space, int precision, int nel, int max_ndof)
 size_t hp3d_gpu_stc_workspace(int nel, int max_ndof, int max_nbubble)
 {
     if (nel <= 0 || max_nbubble <= 0 || max_ndof <= 0) return 0;
-    return (size_t)nel * stc_slab(rnd16(max_nbubble), rnd16(max_ndof + 1)) * sizeof(double);
+    return (size_t)nel * stc_slab(rnd16(max_nbubble)) * sizeof(double);
 }
 
 size_t hp3d_gpu_celem_workspace(int space, int precision, int nel, int max_ndof, int max_nbubble)
@@ -2309,7 +2509,6 @@ int hp3d_gpu_stc_batch(int space, int precision, int nel, const int *norder,
     a.ld_k = ld_k;
     a.ld_c = ld_c;
     a.nbp_max = rnd16(max_nbubble);
-    a.ldy_max = rnd16(max_ndof + 1);
     a.n_max = max_ndof;
     a.nb_max = max_nbubble;
     a.norder = norder;
@@ -2558,7 +2757,6 @@ static int stc_run(int space, int n, int nb, const int *d_norder, const int *d_n
     a.ld_k = 0;
     a.ld_c = 0;
     a.nbp_max = rnd16(g.nbm);
-    a.ldy_max = rnd16(g.max_ndof + 1);
     a.n_max = g.max_ndof;
     a.nb_max = g.nbm;
     a.norder = d_norder;

@@ -1,8 +1,8 @@
 """hp3d_gpu_celem_batch_noalloc (no workspace argument, the library's static
 device scratch) against hp3d_gpu_celem_batch with a caller workspace, through
 the C ABI: every output bit-identical, on two calls in a row, the second with
-a smaller input.  The first call's two-pass bin (H(curl) p = 5, 1.5 MB of
-workspace per element) needs more than one chunk of the scratch."""
+a smaller input.  The first call's two-pass bin (H(curl) p = 6, about 1 MB
+of workspace per element) needs more than one chunk of the scratch."""
 import ctypes
 
 import numpy as np
@@ -58,11 +58,11 @@ def test_noalloc_matches_workspace_entry(gpu):
 
     lib = _lib.load()
     # the first case's two-pass bin needs more than one chunk of the scratch
-    per = lib.hp3d_gpu_celem_workspace(1, 1, 1, 540, 240)
-    assert 0 < per < _lib.SCRATCH_BYTES and 100 > _lib.SCRATCH_BYTES // per
+    per = lib.hp3d_gpu_celem_workspace(1, 1, 1, 882, 450)
+    assert 0 < per < _lib.SCRATCH_BYTES and 150 > _lib.SCRATCH_BYTES // per
     # then a mixed-order H1 mesh (fused bins and a two-pass bin) in FP64,
     # smaller than the first call
-    cases = [(_mesh((5, 5, 4), 5, 71), 1, hp3d_amd.FP32)]
+    cases = [(_mesh((5, 5, 6), 6, 71), 1, hp3d_amd.FP32)]
     rng = np.random.default_rng(72)
     po = rng.choice(np.arange(1, 7), size=(3 * 3 * 2, 3), p=mesh.P_OUT)
     po[:3] = 6
