@@ -1334,7 +1334,7 @@ __host__ __device__ inline size_t stc_slab(int nbp_max)
     return (size_t)(nbk * (nbk + 1) + 2 * nbk) * 256;
 }
 
-template <typename T, int CW, int MAXR, int NW, int RG>
+template <typename T, int CW, int MAXR, int NW, int RG, bool RL>
 __global__ __launch_bounds__(64 * NW) void stc_strip_kernel(StcArgs a)
 {
     typedef Mfma<double> MD;
@@ -1401,6 +1401,12 @@ __global__ __launch_bounds__(64 * NW) void stc_strip_kernel(StcArgs a)
         if (tid == 0 && a.status) a.status[e] = HP3D_E_ARG;
         return;
     }
+    // the register-resident Cholesky holds up to RLB block rows
+    constexpr int RLB = 15;
+    if (RL && nbk > RLB) {
+        if (tid == 0 && a.status) a.status[e] = HP3D_E_ARG;
+        return;
+    }
     const int nt = nbk * (nbk + 1) / 2;
     double *W = a.ws + (size_t)e * stc_slab(a.nbp_max);
     double *Mt = W + (size_t)nt * 256;
@@ -1413,7 +1419,56 @@ __global__ __launch_bounds__(64 * NW) void stc_strip_kernel(StcArgs a)
     // are used: a load whose value is used inside a lane-predicated block is
     // waited for right there, which serialises the loads.
 
-    // ---- 1: Cholesky, left-looking by block panels
+    // the diagonal block p, symmetric in dg: factored in LDS by the calling
+    // wave (lower triangle), L_pp^-1 into li and to the workspace with its
+    // transpose
+    auto factor_diag = [&](int p) {
+            // factor the diagonal block in LDS (lower triangle)
+        for (int jp = 0; jp < 16; ++jp) {
+            if (lane == 0) {
+                double d = dg[jp * 17 + jp];
+                if (!(d > 0.0)) {
+                    s_fail = 1;
+                    d = 1.0;
+                }
+                dg[jp * 17 + jp] = sqrt(d);
+            }
+            wave_sync_lds();
+            if (lane > jp && lane < 16) dg[lane * 17 + jp] /= dg[jp * 17 + jp];
+            wave_sync_lds();
+            for (int idx = lane; idx < 256; idx += 64) {
+                const int i = idx >> 4, m = idx & 15;
+                if (i > jp && m > jp && m <= i) dg[i * 17 + m] -= dg[i * 17 + jp] * dg[m * 17 + jp];
+            }
+            wave_sync_lds();
+        }
+        // L_pp^-1, one column per lane
+        if (lane < 16) {
+            double x[16];
+#pragma unroll
+            for (int m = 0; m < 16; ++m) {
+                // keeps row m's L reads inside the iteration (hoisted,
+                // the 136 reads would take 272 VGPRs)
+                __asm__ volatile("" ::: "memory");
+                double s = m == lane ? 1.0 : 0.0;
+#pragma unroll
+                for (int k = 0; k < m; ++k) s -= dg[m * 17 + k] * x[k];
+                x[m] = m < lane ? 0.0 : s / dg[m * 17 + m];
+            }
+#pragma unroll
+            for (int m = 0; m < 16; ++m) li[m * 17 + lane] = x[m];
+        }
+        wave_sync_lds();
+        for (int idx = lane; idx < 256; idx += 64) {
+            const int r = idx & 15, c = idx >> 4; // column-major tiles
+            Linv[(size_t)p * 256 + idx] = li[r * 17 + c];
+            LinvT[(size_t)p * 256 + idx] = li[c * 17 + r];
+        }
+    };
+
+    if constexpr (!RL) {
+    // ---- 1: Cholesky, left-looking by block panels (more block rows than
+    // the registers can hold)
     for (int p = 0; p < nbk; ++p) {
         // A(i,p) of Abb (K symmetric: read along rows), padded with the identity
         T araw[MAXR][4];
@@ -1480,51 +1535,7 @@ __global__ __launch_bounds__(64 * NW) void stc_strip_kernel(StcArgs a)
                 for (int j = 0; j < 4; ++j) dg[MD::row(lane, j) * 17 + lr] = cacc[r][j];
             }
         __syncthreads();
-        if (wave == p % NW) {
-            // factor the diagonal block in LDS (lower triangle)
-            for (int jp = 0; jp < 16; ++jp) {
-                if (lane == 0) {
-                    double d = dg[jp * 17 + jp];
-                    if (!(d > 0.0)) {
-                        s_fail = 1;
-                        d = 1.0;
-                    }
-                    dg[jp * 17 + jp] = sqrt(d);
-                }
-                wave_sync_lds();
-                if (lane > jp && lane < 16) dg[lane * 17 + jp] /= dg[jp * 17 + jp];
-                wave_sync_lds();
-                for (int idx = lane; idx < 256; idx += 64) {
-                    const int i = idx >> 4, m = idx & 15;
-                    if (i > jp && m > jp && m <= i) dg[i * 17 + m] -= dg[i * 17 + jp] * dg[m * 17 + jp];
-                }
-                wave_sync_lds();
-            }
-            // L_pp^-1, one column per lane
-            if (lane < 16) {
-                double x[16];
-#pragma unroll
-                for (int m = 0; m < 16; ++m) {
-                    // keeps row m's L reads inside the iteration (hoisted,
-                    // the 136 reads would take 272 VGPRs)
-                    __asm__ volatile("" ::: "memory");
-                    double s = m == lane ? 1.0 : 0.0;
-#pragma unroll
-                    for (int k = 0; k < m; ++k) s -= dg[m * 17 + k] * x[k];
-                    x[m] = m < lane ? 0.0 : s / dg[m * 17 + m];
-                }
-#pragma unroll
-                for (int m = 0; m < 16; ++m) li[m * 17 + lane] = x[m];
-            }
-            wave_sync_lds();
-            double *Lpp = W + (size_t)stc_tl(p, p) * 256;
-            for (int idx = lane; idx < 256; idx += 64) {
-                const int r = idx & 15, c = idx >> 4; // column-major tiles
-                Lpp[idx] = r >= c ? dg[r * 17 + c] : 0.0;
-                Linv[(size_t)p * 256 + idx] = li[r * 17 + c];
-                LinvT[(size_t)p * 256 + idx] = li[c * 17 + r];
-            }
-        }
+        if (wave == p % NW) factor_diag(p);
         __syncthreads();
         // the panel below: L(i,p) = A~(i,p) L_pp^-T
 #pragma unroll
@@ -1546,6 +1557,95 @@ __global__ __launch_bounds__(64 * NW) void stc_strip_kernel(StcArgs a)
             for (int j = 0; j < 4; ++j) Lip[MD::row(lane, j) + 16 * lr] = d[j];
         }
         __syncthreads();
+    }
+    } else {
+    // ---- 1: Cholesky, right-looking, the lower triangle of Abb held in
+    // registers as 16 x 16 tiles (tile t = (i, j) of the lower triangle by
+    // wave t mod 8): per block column p the owner of the diagonal tile
+    // factors it, the owners of the tiles below form the panel
+    // L(i,p) = A~(i,p) L_pp^-T and broadcast it through LDS, and every wave
+    // updates its trailing tiles A~(i,j) -= L(i,p) L(j,p)^T
+    constexpr int NSL = (RLB * (RLB + 1) / 2 + NW - 1) / NW; // tile slots per wave
+    d4_t tl[NSL];
+    // (i, j) of slot u's tile t = wave + 8u, walked incrementally
+    auto slot_ij = [&](auto&& f) {
+        int i = 0, j = wave;
+        while (j > i) {
+            j -= i + 1;
+            ++i;
+        }
+#pragma unroll
+        for (int u = 0; u < NSL; ++u) {
+            f(u, wave + NW * u < nt ? i : -1, j);
+            j += NW;
+            while (j > i) {
+                j -= i + 1;
+                ++i;
+            }
+        }
+    };
+    // A(i,j) of Abb (K symmetric: read along rows), loaded without
+    // conditions, then the padding set to the identity
+    slot_ij([&](int u, int i, int j) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int rr = 16 * i + MD::row(lane, q), cc = 16 * j + lr;
+            const bool v = i >= 0 && rr < nb && cc < nb;
+            tl[u][q] = (double)K[v ? (size_t)(ni + cc) + (size_t)(ni + rr) * ldk : 0];
+        }
+    });
+    slot_ij([&](int u, int i, int j) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int rr = 16 * i + MD::row(lane, q), cc = 16 * j + lr;
+            if (!(i >= 0 && rr < nb && cc < nb)) tl[u][q] = rr == cc ? 1.0 : 0.0;
+        }
+    });
+    double *pan = xs; // the panel's tiles L(i,p), column-major, at pan + 256 i
+    for (int p = 0; p < nbk; ++p) {
+        if (wave == stc_tl(p, p) % NW) {
+            slot_ij([&](int u, int i, int j) {
+                if (i == p && j == p) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) dg[MD::row(lane, q) * 17 + lr] = tl[u][q];
+                }
+            });
+            wave_sync_lds();
+            factor_diag(p);
+        }
+        lds_barrier();
+        slot_ij([&](int u, int i, int j) {
+            if (j == p && i > p) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) st[MD::row(lane, q) * 17 + lr] = tl[u][q];
+                wave_sync_lds();
+                d4_t d = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int s2 = 0; s2 < 4; ++s2) {
+                    const int kk = 4 * s2 + lq;
+                    d = MD::mma(st[lr * 17 + kk], li[lr * 17 + kk], d);
+                }
+                wave_sync_lds();
+                double *Lip = W + (size_t)stc_tl(i, p) * 256, *Pp = pan + (size_t)256 * i;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    Lip[MD::row(lane, q) + 16 * lr] = d[q];
+                    Pp[MD::row(lane, q) + 16 * lr] = d[q];
+                }
+            }
+        });
+        lds_barrier();
+        slot_ij([&](int u, int i, int j) {
+            if (i >= 0 && j > p) {
+                const double *Pi = pan + (size_t)256 * i, *Pj = pan + (size_t)256 * j;
+                d4_t sum = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int s2 = 0; s2 < 4; ++s2) sum = MD::mma(Pi[ab + 64 * s2], Pj[ab + 64 * s2], sum);
+                tl[u] -= sum;
+            }
+        });
+    }
+    __syncthreads();
     }
 
     // ---- the sweeps' tiles: -M(i,p) = -L(i,p) L_pp^-1 (forward) and
@@ -1636,7 +1736,8 @@ __global__ __launch_bounds__(64 * NW) void stc_strip_kernel(StcArgs a)
         // subtracts T(i,p) row_p from its rows i (forward: i > p, T = M;
         // backward: i < p, T = Mb(p,i)^T) and the owner of the next row
         // updates and broadcasts it first; the tiles of step p + 1 are loaded
-        // during step p
+        // during step p.  No row is needed solved during the sweep: each wave
+        // solves its own rows at the end, against their diagonal tiles.
         auto sweep = [&](bool fwd) {
             double nL[MAXR][4];
             auto ld_step = [&](int p) {
@@ -1651,17 +1752,18 @@ __global__ __launch_bounds__(64 * NW) void stc_strip_kernel(StcArgs a)
             };
             const int p0 = fwd ? 0 : nbk - 1, dp = fwd ? 1 : -1;
             const double *Dg = fwd ? Linv : LinvT;
-            double dA[4]; // the diagonal tile of the coming step's row
-            ld_step(p0);
+            double dS[MAXR][4]; // diagonal tiles of this wave's rows
 #pragma unroll
-            for (int s2 = 0; s2 < 4; ++s2) dA[s2] = Dg[(size_t)p0 * 256 + ab + 64 * s2];
+            for (int r = 0; r < MAXR; ++r) {
+                const int i = min(wave + NW * r, nbk - 1);
+#pragma unroll
+                for (int s2 = 0; s2 < 4; ++s2) dS[r][s2] = Dg[(size_t)i * 256 + ab + 64 * s2];
+            }
+            ld_step(p0);
             if (wave == p0 % NW) {
 #pragma unroll
                 for (int r = 0; r < MAXR; ++r)
-                    if (r == p0 / NW) {
-                        put_row(acc[r], p0);
-                        solve_row(acc[r], dA, p0);
-                    }
+                    if (r == p0 / NW) put_row(acc[r], p0);
             }
             for (int p = p0; p >= 0 && p < nbk; p += dp) {
                 double cL[MAXR][4];
@@ -1671,10 +1773,7 @@ __global__ __launch_bounds__(64 * NW) void stc_strip_kernel(StcArgs a)
                     for (int s2 = 0; s2 < 4; ++s2) cL[r][s2] = nL[r][s2];
                 const int pn = p + dp;
                 const bool more = pn >= 0 && pn < nbk;
-                const int pc = more ? pn : p; // loads stay unconditional
-                ld_step(pc);
-#pragma unroll
-                for (int s2 = 0; s2 < 4; ++s2) dA[s2] = Dg[(size_t)pc * 256 + ab + 64 * s2];
+                ld_step(more ? pn : p); // loads stay unconditional
                 const bool own_next = more && wave == pn % NW;
                 lds_barrier();
                 const double *buf = xs + (size_t)16 * p * STR;
@@ -1686,13 +1785,12 @@ __global__ __launch_bounds__(64 * NW) void stc_strip_kernel(StcArgs a)
                             acc[r][t] = MD::mma(cL[r][s2], buf[(4 * s2 + lq) * STR + 16 * t + lr], acc[r][t]);
                     }
                 };
-                // the next row first: update, broadcast, solve
+                // the next row first: update, broadcast
 #pragma unroll
                 for (int r = 0; r < MAXR; ++r)
                     if (own_next && r == pn / NW) {
                         upd(r);
                         put_row(acc[r], pn);
-                        solve_row(acc[r], dA, pn);
                     }
 #pragma unroll
                 for (int r = 0; r < MAXR; ++r) {
@@ -1702,6 +1800,10 @@ __global__ __launch_bounds__(64 * NW) void stc_strip_kernel(StcArgs a)
                     upd(r);
                 }
             }
+            // the wave's own rows: their final broadcast values in LDS, solved
+#pragma unroll
+            for (int r = 0; r < MAXR; ++r)
+                if (wave + NW * r < nbk) solve_row(acc[r], dS[r], wave + NW * r);
         };
         sweep(true);   // S = L^-1 Z (rows of S in registers)
         lds_barrier(); // every wave has read the forward rows
@@ -1784,7 +1886,8 @@ __global__ __launch_bounds__(64 * NW) void stc_strip_kernel(StcArgs a)
                     for (int t = 0; t < CT; ++t) {
                         const double bv = xs[(size_t)(16 * p + 4 * s2 + lq) * STR + 16 * t + lr];
 #pragma unroll
-                        for (int g = 0; g < RG; ++g) accA[g][t] = MD::mma(cA[g][s2], bv, accA[g][t]);
+                        for (int g = 0; g < RG; ++g)
+                            if (I0 + NW * g < nI) accA[g][t] = MD::mma(cA[g][s2], bv, accA[g][t]);
                     }
                 }
             }
@@ -1840,12 +1943,16 @@ __global__ __launch_bounds__(64 * NW) void stc_strip_kernel(StcArgs a)
 template <typename T>
 static void launch_stc(StcArgs a, int max_nbubble, hipStream_t st)
 {
-    // 64-column strips (2 block rows per wave) up to 256 bubbles, 32-column
-    // strips (4 block rows per wave) beyond
-    if (rnd16(max_nbubble) <= 256)
-        hipLaunchKernelGGL((stc_strip_kernel<T, 64, 2, 8, 2>), dim3(a.nel), dim3(512), 0, st, a);
+    // 64-column strips (2 block rows per wave) up to 256 bubbles, with the
+    // register-resident Cholesky up to 240; 32-column strips (4 block rows
+    // per wave) beyond
+    const int nbp = rnd16(max_nbubble);
+    if (nbp <= 240)
+        hipLaunchKernelGGL((stc_strip_kernel<T, 64, 2, 8, 2, true>), dim3(a.nel), dim3(512), 0, st, a);
+    else if (nbp <= 256)
+        hipLaunchKernelGGL((stc_strip_kernel<T, 64, 2, 8, 2, false>), dim3(a.nel), dim3(512), 0, st, a);
     else
-        hipLaunchKernelGGL((stc_strip_kernel<T, 32, 4, 8, 2>), dim3(a.nel), dim3(512), 0, st, a);
+        hipLaunchKernelGGL((stc_strip_kernel<T, 32, 4, 8, 2, false>), dim3(a.nel), dim3(512), 0, st, a);
 }
 
 struct BwdArgs {
