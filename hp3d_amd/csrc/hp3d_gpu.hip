@@ -1343,7 +1343,7 @@ __global__ __launch_bounds__(64 * NW) void stc_strip_kernel(StcArgs a)
     constexpr int STR = CW + 2;         // LDS row stride of the strip
     constexpr int NBPX = 16 * NW * MAXR; // largest padded bubble block
     __shared__ double xs[NBPX * STR];   // the strip: solved block rows as they are broadcast
-    __shared__ double dg[16 * 17], li[16 * 17];
+    __shared__ double dg[16 * 17], li[16 * 17], rsd[16];
     __shared__ double stg[NW][16 * 17];
     __shared__ int s_n, s_nb, s_fail;
     const int e = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
@@ -1419,44 +1419,17 @@ __global__ __launch_bounds__(64 * NW) void stc_strip_kernel(StcArgs a)
     // are used: a load whose value is used inside a lane-predicated block is
     // waited for right there, which serialises the loads.
 
-    // the diagonal block p, symmetric in dg: factored in LDS by the calling
-    // wave (lower triangle), L_pp^-1 into li and to the workspace with its
-    // transpose
+    // the diagonal block p, symmetric in dg: factored by the calling wave,
+    // L_pp^-1 into li and to the workspace with its transpose
     auto factor_diag = [&](int p) {
-            // factor the diagonal block in LDS (lower triangle)
-        for (int jp = 0; jp < 16; ++jp) {
-            if (lane == 0) {
-                double d = dg[jp * 17 + jp];
-                if (!(d > 0.0)) {
-                    s_fail = 1;
-                    d = 1.0;
-                }
-                dg[jp * 17 + jp] = sqrt(d);
-            }
-            wave_sync_lds();
-            if (lane > jp && lane < 16) dg[lane * 17 + jp] /= dg[jp * 17 + jp];
-            wave_sync_lds();
-            for (int idx = lane; idx < 256; idx += 64) {
-                const int i = idx >> 4, m = idx & 15;
-                if (i > jp && m > jp && m <= i) dg[i * 17 + m] -= dg[i * 17 + jp] * dg[m * 17 + jp];
-            }
-            wave_sync_lds();
-        }
-        // L_pp^-1, one column per lane
-        if (lane < 16) {
-            double x[16];
-#pragma unroll
-            for (int m = 0; m < 16; ++m) {
-                // keeps row m's L reads inside the iteration (hoisted,
-                // the 136 reads would take 272 VGPRs)
-                __asm__ volatile("" ::: "memory");
-                double s = m == lane ? 1.0 : 0.0;
-#pragma unroll
-                for (int k = 0; k < m; ++k) s -= dg[m * 17 + k] * x[k];
-                x[m] = m < lane ? 0.0 : s / dg[m * 17 + m];
-            }
-#pragma unroll
-            for (int m = 0; m < 16; ++m) li[m * 17 + lane] = x[m];
+        // Abb_pp = L_u D L_u^T by the one-wave elimination of [Abb_pp | I]
+        // (L_u^-1 into li, D^-1/2 into rsd), then L_pp^-1 = D^-1/2 L_u^-1
+        const bool bad = eliminate_wave<16>(dg, 17, 0, 16, li, 17, rsd, lane);
+        if (bad && lane == 0) s_fail = 1;
+        wave_sync_lds();
+        for (int idx = lane; idx < 256; idx += 64) {
+            const int m = idx >> 4, c = idx & 15;
+            li[m * 17 + c] *= rsd[m];
         }
         wave_sync_lds();
         for (int idx = lane; idx < 256; idx += 64) {
