@@ -15,6 +15,7 @@ H1, HCURL = 0, 1
 FP64, FP32, BF16 = 0, 1, 2
 LOAD_ZERO, LOAD_SINSINSIN, LOAD_CONV, LOAD_ONE, LOAD_CURL_SIN = 0, 1, 2, 3, 10
 LOAD_POLY2 = 4
+LOAD_XYZ = 5
 FIELD_ERROR, FIELD_ADJOINT, FIELD_L2 = 0, 1, 2
 OPT_PERSISTENT = 1
 SCRATCH_BYTES = 128 << 20   # HP3D_SCRATCH_BYTES
@@ -51,6 +52,8 @@ SIGNATURES = {
     "hp3d_gpu_stc_bwd_batch": (_i, [_i, _i, _p, _p, _p, _p, _p, _p, _p]),
     "hp3d_gpu_field_batch": (_i, [_i, _i, _i, _p, _p, _p, _p, _i, _p, _p, _p, _p, _i, _p, _p,
                                   _p, _p]),
+    "hp3d_gpu_dirichlet_lift": (_i, [_i, _i, _p, _p, _p, _p, _p, _p, _p]),
+    "hp3d_gpu_shape": (_i, [_i, _p, _p, _p, _i, _p, _p, _p]),
     "hp3d_gpu_gather_sum": (_i, [ctypes.c_longlong, _p, _p, _p, _p, _p]),
     "hp3d_gpu_csr_spmv": (_i, [ctypes.c_longlong, _p, _p, _p, _p, _p, _p]),
 }

@@ -11,6 +11,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FC = os.environ.get("FC_HP3D", "/opt/rocm/bin/amdflang")
 FDIR = os.path.join(HERE, "fortran")
 FDRIVER = os.path.join(FDIR, "elem_driver.bin")
+ROCM_LIB = "/opt/rocm/lib"
 
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
@@ -41,7 +42,9 @@ def build_fortran():
     os.makedirs(mod, exist_ok=True)
     cmd = [FC, "-O2", "-J", mod, "-o", FDRIVER + ".tmp",
            os.path.join(FDIR, "hp3d_gpu_mod.F90"), os.path.join(FDIR, "elem_driver.F90"),
-           "-L" + HERE, "-lhp3d_gpu", "-Wl,-rpath,$ORIGIN/.."]
+           "-L" + HERE, "-lhp3d_gpu", "-Wl,-rpath,$ORIGIN/..",
+           # the batched modes stage device memory with the HIP runtime
+           "-L" + ROCM_LIB, "-lamdhip64", "-Wl,-rpath," + ROCM_LIB]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("amdflang failed:\n" + r.stdout + r.stderr)

@@ -1955,6 +1955,37 @@ __global__ __launch_bounds__(WG) void stc_bwd_kernel(BwdArgs a)
     }
 }
 
+// Dirichlet lift of the condensed element loads (celem_systemI.F90:794-808,
+// ZBMOD(:) -= ZAMOD(:,k2) ZDOFD(k2) for every Dirichlet dof k2), on the
+// condensed system: bi[k] -= sum_k2 Aii[k, k2] ud[k2], with ud the element's
+// interface vector of Dirichlet values (0 at free dofs).  One workgroup per
+// element, a thread per row.
+struct LiftArgs {
+    int space, nel;
+    const int *norder;
+    const long long *off_c, *off_v;
+    const double *aii, *ud;
+    double *bi;
+};
+
+__global__ __launch_bounds__(WG) void dirichlet_lift_kernel(LiftArgs a)
+{
+    const int e = blockIdx.x;
+    int zero[18] = {0};
+    ElemOrders o;
+    if (!decode_orders(a.norder + 19 * (size_t)e, zero, zero + 12, o)) return;
+    int n, nb;
+    dof_counts(a.space, o, n, nb);
+    const int ni = n - nb;
+    const double *A = a.aii + a.off_c[e], *u = a.ud + a.off_v[e];
+    double *b = a.bi + a.off_v[e];
+    for (int k = threadIdx.x; k < ni; k += WG) {
+        double s = 0.0;
+        for (int k2 = 0; k2 < ni; ++k2) s += A[(size_t)k2 * ni + k] * u[k2];
+        b[k] -= s;
+    }
+}
+
 // ------------------------------------------------------- field evaluation
 //
 // One workgroup per element evaluates finite-element fields from element dof
@@ -2008,6 +2039,11 @@ __device__ inline void exact_h1(int sel, const double *x, double &u, double *du)
         du[0] = (1.0 - 2.0 * x[0]) * a1 * a2;
         du[1] = a0 * (1.0 - 2.0 * x[1]) * a2;
         du[2] = a0 * a1 * (1.0 - 2.0 * x[2]);
+    } else if (sel == HP3D_LOAD_XYZ) {
+        u = x[0] * x[1] * x[2];
+        du[0] = x[1] * x[2];
+        du[1] = x[0] * x[2];
+        du[2] = x[0] * x[1];
     }
 }
 
@@ -2718,6 +2754,54 @@ int hp3d_gpu_stc_bwd_batch(int space, int nel, const int *norder, const long lon
     BwdArgs a{space, nel, norder, off_s, off_v, aschur, bschur, x};
     hipLaunchKernelGGL(stc_bwd_kernel, dim3(nel), dim3(WG), 0, (hipStream_t)stream, a);
     HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int hp3d_gpu_dirichlet_lift(int space, int nel, const int *norder, const long long *off_c,
+                            const double *aii, const long long *off_v, const double *ud, double *bi,
+                            void *stream)
+{
+    if ((space != HP3D_H1 && space != HP3D_HCURL) || nel < 0) return HP3D_E_ARG;
+    if (nel == 0) return 0;
+    if (!norder || !off_c || !aii || !off_v || !ud || !bi) return HP3D_E_ARG;
+    LiftArgs a{space, nel, norder, off_c, off_v, aii, ud, bi};
+    hipLaunchKernelGGL(dirichlet_lift_kernel, dim3(nel), dim3(WG), 0, (hipStream_t)stream, a);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int hp3d_gpu_shape(int space, const int norder[19], const int nori_e[12], const int nori_f[6],
+                   int npts, const double *xi, double *val, double *grad)
+{
+    if ((space != HP3D_H1 && space != HP3D_HCURL) || npts < 0) return HP3D_E_ARG;
+    if (!norder || !nori_e || !nori_f || (npts > 0 && (!xi || !val || !grad))) return HP3D_E_ARG;
+    ElemOrders o;
+    if (!decode_orders(norder, nori_e, nori_f, o)) return HP3D_E_ORDER;
+    int n, nb;
+    dof_counts(space, o, n, nb);
+    for (int p = 0; p < npts; ++p) {
+        double tv[3][NTAB], td[3][NTAB];
+        for (int d = 0; d < 3; ++d) table_at(xi[3 * p + d], tv[d], td[d]);
+        for (int k = 0; k < n; ++k) {
+            const unsigned dsc = space == HP3D_H1 ? desc_h1(k, o) : desc_hcurl(k, o);
+            const int i0 = desc_ix(dsc, 0), i1 = desc_ix(dsc, 1), i2 = desc_ix(dsc, 2);
+            const double v0 = tv[0][i0], v1 = tv[1][i1], v2 = tv[2][i2];
+            const double gr[3] = {td[0][i0] * v1 * v2, v0 * td[1][i1] * v2, v0 * v1 * td[2][i2]};
+            const size_t pk = (size_t)p * n + k;
+            if (space == HP3D_H1) {
+                val[pk] = v0 * v1 * v2;
+                for (int c = 0; c < 3; ++c) grad[3 * pk + c] = gr[c];
+            } else {
+                // one component cp of sign sg: E = sg phi e_cp, curl E = sg grad phi x e_cp
+                const int cp = desc_comp(dsc);
+                const double sg = desc_neg(dsc) ? -1.0 : 1.0;
+                for (int c = 0; c < 3; ++c) val[3 * pk + c] = c == cp ? sg * v0 * v1 * v2 : 0.0;
+                grad[3 * pk + 0] = cp == 0 ? 0.0 : (cp == 1 ? -sg * gr[2] : sg * gr[1]);
+                grad[3 * pk + 1] = cp == 0 ? sg * gr[2] : (cp == 1 ? 0.0 : -sg * gr[0]);
+                grad[3 * pk + 2] = cp == 0 ? -sg * gr[1] : (cp == 1 ? sg * gr[0] : 0.0);
+            }
+        }
+    }
     return 0;
 }
 

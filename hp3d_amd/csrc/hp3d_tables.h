@@ -300,4 +300,40 @@ HP3D_HD inline int desc_ix(unsigned d, int dir) { return (d >> (5 * dir)) & 31; 
 HP3D_HD inline int desc_comp(unsigned d) { return (d >> 15) & 3; }
 HP3D_HD inline int desc_neg(unsigned d) { return (d >> 17) & 1; }
 
+// The NTAB entries of the 1D factor table at one coordinate x in [0, 1]
+// (values v, d/dx dv), with the formulas and evaluation order of the
+// kernels' table fill: 1 - x and x; phi_i(s) = (P_i(2s-1) - P_{i-2}(2s-1)) /
+// (4i - 2) with d phi_i/ds = P_{i-1}(2s-1), s = x or 1 - x (PolyILegendre,
+// Idec = .true.); the scaled Legendre P_i(s; t), t = (1 - x) + x, of the
+// oriented pair (HomLegendre).
+HP3D_HD inline void table_at(double x, double *v, double *dv)
+{
+    v[0] = 1.0 - x;
+    dv[0] = -1.0;
+    v[1] = x;
+    dv[1] = 1.0;
+    for (int flip = 0; flip < 2; ++flip) {
+        double s = flip ? 1.0 - x : x;
+        double y = 2.0 * s - 1.0, p[MAXP + 1];
+        p[0] = 1.0;
+        p[1] = y;
+        for (int i = 2; i <= MAXP; ++i)
+            p[i] = ((double)(2 * i - 1) * y * p[i - 1] - (double)(i - 1) * p[i - 2]) / (double)i;
+        for (int i = 2; i <= MAXP; ++i) {
+            v[t_phi(i, flip)] = (p[i] - p[i - 2]) / (double)(4 * i - 2);
+            dv[t_phi(i, flip)] = flip ? -p[i - 1] : p[i - 1];
+        }
+        double t = (1.0 - x) + x;
+        double yy = 2.0 * s - t, tt = t * t, r[MAXP];
+        r[0] = 1.0;
+        r[1] = yy;
+        for (int i = 2; i < MAXP; ++i)
+            r[i] = ((double)(2 * i - 1) * yy * r[i - 1] - (double)(i - 1) * tt * r[i - 2]) / (double)i;
+        for (int i = 0; i < MAXP; ++i) {
+            v[t_leg(i, flip)] = r[i];
+            dv[t_leg(i, flip)] = 0.0;
+        }
+    }
+}
+
 } // namespace hp3d

@@ -22,9 +22,11 @@ module hp3d_gpu_mod
   integer(c_int), parameter :: HP3D_H1 = 0, HP3D_HCURL = 1
   integer(c_int), parameter :: HP3D_FP64 = 0, HP3D_FP32 = 1, HP3D_BF16 = 2
   integer(c_int), parameter :: HP3D_LOAD_ZERO = 0, HP3D_LOAD_SINSINSIN = 1, HP3D_LOAD_CONV = 2
-  integer(c_int), parameter :: HP3D_LOAD_ONE = 3, HP3D_LOAD_POLY2 = 4, HP3D_LOAD_CURL_SIN = 10
+  integer(c_int), parameter :: HP3D_LOAD_ONE = 3, HP3D_LOAD_POLY2 = 4, HP3D_LOAD_XYZ = 5
+  integer(c_int), parameter :: HP3D_LOAD_CURL_SIN = 10
   integer(c_int), parameter :: HP3D_FIELD_ERROR = 0, HP3D_FIELD_ADJOINT = 1, HP3D_FIELD_L2 = 2
   integer(c_int), parameter :: HP3D_OPT_PERSISTENT = 1
+  integer(c_size_t), parameter :: HP3D_SCRATCH_BYTES = 134217728_c_size_t
 
   interface
     integer(c_int) function hp3d_gpu_init(device, max_ndof) bind(C, name="hp3d_gpu_init")
@@ -126,6 +128,32 @@ module hp3d_gpu_mod
       integer(c_int), value :: space, precision, nel, ngeo, load, ld_k, max_ndof, max_nbubble
       type(c_ptr), value :: norder, nori_e, nori_f, xnod, coef, off_k, kout, off_v, bout
       type(c_ptr), value :: off_c, aii, bi, off_s, aschur, bschur, workspace, status, stream
+    end function
+
+    integer(c_int) function hp3d_gpu_celem_batch_noalloc(space, precision, nel, norder, nori_e, &
+        nori_f, xnod, ngeo, coef, load, off_k, ld_k, kout, off_v, bout, off_c, aii, bi, off_s, &
+        aschur, bschur, status, max_ndof, max_nbubble, stream) &
+        bind(C, name="hp3d_gpu_celem_batch_noalloc")
+      import :: c_int, c_ptr
+      integer(c_int), value :: space, precision, nel, ngeo, load, ld_k, max_ndof, max_nbubble
+      type(c_ptr), value :: norder, nori_e, nori_f, xnod, coef, off_k, kout, off_v, bout
+      type(c_ptr), value :: off_c, aii, bi, off_s, aschur, bschur, status, stream
+    end function
+
+    integer(c_int) function hp3d_gpu_dirichlet_lift(space, nel, norder, off_c, aii, off_v, ud, &
+        bi, stream) bind(C, name="hp3d_gpu_dirichlet_lift")
+      import :: c_int, c_ptr
+      integer(c_int), value :: space, nel
+      type(c_ptr), value :: norder, off_c, aii, off_v, ud, bi, stream
+    end function
+
+    integer(c_int) function hp3d_gpu_shape(space, norder, nori_e, nori_f, npts, xi, val, grad) &
+        bind(C, name="hp3d_gpu_shape")
+      import :: c_int, c_double
+      integer(c_int), value :: space, npts
+      integer(c_int), intent(in) :: norder(19), nori_e(12), nori_f(6)
+      real(c_double), intent(in) :: xi(*)
+      real(c_double), intent(out) :: val(*), grad(*)
     end function
 
     integer(c_int) function hp3d_gpu_stc_bwd_batch(space, nel, norder, off_s, aschur, off_v, &

@@ -52,9 +52,18 @@ class DofMap:
 
     gid[e, :ni_e] are the global numbers of element e's interface dofs in
     element order (vertices, edges, faces), -1 for a Dirichlet dof; columns
-    beyond ni_e hold -2."""
+    beyond ni_e hold -2.  node_kind[e, k] (0 vertex, 1 edge, 2 face), node_id
+    and node_j give the node each dof belongs to and its index among the
+    node's dofs (equal across the elements sharing the node: trace
+    conformity).
 
-    def __init__(self, mesh, space=H1, dirichlet=True):
+    numbering="blocks" numbers the free dofs node by node, vertices, then
+    edges, then faces, each by global node id; numbering="first_visit" is
+    hp3D's NFIRST_DOF: the nodes in the order the element loop first visits
+    them (elements in mesh order, each element's vertices, edges, faces),
+    mumps_sc.F90:150-158."""
+
+    def __init__(self, mesh, space=H1, dirichlet=True, numbering="blocks"):
         self.space = space
         m = mesh
         nel = m.nel
@@ -78,43 +87,90 @@ class DofMap:
         if el.size:
             bedge[m.elem_edges[el[:, None], FACE_EDGES[lf]].ravel()] = True
             bvert[m.elems[el[:, None], FACE_VERTS[lf]].ravel()] = True
+        self.boundary = (bvert, bedge, bface)
         if not dirichlet:
-            bface[:], bedge[:], bvert[:] = False, False, False
+            bface, bedge, bvert = bface & False, bedge & False, bvert & False
+        self.fixed = (bvert, bedge, bface)
         counts = np.concatenate([np.where(bvert, 0, nvd), np.where(bedge, 0, ned_d),
                                  np.where(bface, 0, nfd)])
-        base = np.concatenate([[0], np.cumsum(counts)[:-1]])
+        if numbering == "blocks":
+            base = np.concatenate([[0], np.cumsum(counts)[:-1]])
+        elif numbering == "first_visit":
+            # node keys (vertices, then edges, then faces in one index space)
+            # in the order the element loop visits them
+            visit = np.concatenate([m.elems, nv + m.elem_edges, nv + ned + m.elem_faces],
+                                   axis=1).ravel()
+            _, first = np.unique(visit, return_index=True)
+            order = visit[np.sort(first)]              # nodes by first visit
+            base = np.zeros(counts.size, np.int64)
+            base[order] = np.concatenate([[0], np.cumsum(counts[order])[:-1]])
+        else:
+            raise ValueError("numbering must be 'blocks' or 'first_visit'")
+        self.numbering = numbering
         self.ndof = int(counts.sum())
         vbase, ebase, fbase = base[:nv], base[nv:nv + ned], base[nv + ned:]
         vfix, efix, ffix = bvert, bedge, bface
-        # per-element blocks (node, count) in element order
+        # per-element blocks (kind, node, base, count, fixed) in element order
         blocks = []
         if space == H1:
             for lv in range(8):
                 g = m.elems[:, lv]
-                blocks.append((vbase[g], np.ones(nel, np.int64), vfix[g]))
+                blocks.append((0, g, vbase[g], np.ones(nel, np.int64), vfix[g]))
         no = m.norder
         for le in range(12):
             g = m.elem_edges[:, le]
             c = no[:, le] - 1 if space == H1 else no[:, le].astype(np.int64)
-            blocks.append((ebase[g], c.astype(np.int64), efix[g]))
+            blocks.append((1, g, ebase[g], c.astype(np.int64), efix[g]))
         for lf in range(6):
             g = m.elem_faces[:, lf]
             h, v = no[:, 12 + lf] // 10, no[:, 12 + lf] % 10
             c = (h - 1) * (v - 1) if space == H1 else h * (v - 1) + (h - 1) * v
-            blocks.append((fbase[g], c.astype(np.int64), ffix[g]))
-        ni = np.sum([b[1] for b in blocks], axis=0)
+            blocks.append((2, g, fbase[g], c.astype(np.int64), ffix[g]))
+        ni = np.sum([b[3] for b in blocks], axis=0)
         self.ni = ni
         width = int(ni.max()) if nel else 0
         gid = np.full((nel, width), -2, np.int64)
+        kind = np.full((nel, width), -1, np.int8)
+        nid = np.full((nel, width), -1, np.int64)
+        nj = np.full((nel, width), -1, np.int64)
         pos = np.zeros(nel, np.int64)
         rows = np.arange(nel)
-        for b0, c, fixed in blocks:
+        for knd, node, b0, c, fixed in blocks:
             for k in range(int(c.max()) if nel else 0):
                 sel = k < c
                 r = rows[sel]
-                gid[r, pos[sel] + k] = np.where(fixed[sel], -1, b0[sel] + k)
+                col = pos[sel] + k
+                gid[r, col] = np.where(fixed[sel], -1, b0[sel] + k)
+                kind[r, col] = knd
+                nid[r, col] = node[sel]
+                nj[r, col] = k
             pos += c
         self.gid = gid
+        self.node_kind, self.node_id, self.node_j = kind, nid, nj
+
+    def element_system(self, e, aii, bi, ud=None):
+        """Element e's condensed system in the layout celem_systemI hands to
+        the solver (celem_systemI.F90:285-330, 794-843, with ISYM_FLAG = 2):
+        the Dirichlet lift Bi - Aii ud, then the free dofs only, nodes in
+        reverse element order (faces 6..1, edges 12..1, vertices 8..1), each
+        node's dofs in increasing order (NEXTRACT).  Returns (Zastif row-major
+        nrdofc x nrdofc as a flat array, Zbload, LCON) with LCON the 1-based
+        global numbers (mumps_sc.F90:266-283).  aii is ni x ni, bi length ni
+        (host arrays, element order)."""
+        ni = int(self.ni[e])
+        g = self.gid[e, :ni]
+        kind, node = self.node_kind[e, :ni], self.node_id[e, :ni]
+        b = np.asarray(bi, dtype=np.float64)[:ni].copy()
+        a = np.asarray(aii, dtype=np.float64)[:ni, :ni]
+        if ud is not None:
+            b -= a @ np.asarray(ud, dtype=np.float64)[:ni]
+        # node blocks of the element in element order, then reversed
+        starts = np.nonzero(np.r_[True, (kind[1:] != kind[:-1]) | (node[1:] != node[:-1])])[0]
+        ends = np.r_[starts[1:], ni]
+        nextract = np.concatenate([np.arange(s0, s1) for s0, s1 in zip(starts[::-1], ends[::-1])])
+        nextract = nextract[g[nextract] >= 0]
+        zastif = a[np.ix_(nextract, nextract)].reshape(-1)
+        return zastif, b[nextract], g[nextract] + 1
 
 
 class Assembly:
@@ -342,10 +398,41 @@ def field(batch, mode, u, v=None, exact=0, with_r=False):
     return (out.view(batch.nel, 2) if width == 2 else out), r
 
 
+def dirichlet_lift(batch, out, ud):
+    """Condensed element loads minus Aii ud on the device
+    (hp3d_gpu_dirichlet_lift; ud packed like out['bi'], internal order)."""
+    import torch
+
+    lib = _lib.load()
+    st = torch.cuda.current_stream().cuda_stream
+    p = lambda t: ctypes.c_void_p(t.data_ptr())
+    rc = lib.hp3d_gpu_dirichlet_lift(batch.space, batch.nel, p(batch.norder), p(batch.off_c),
+                                     p(out["aii"]), p(batch.off_v), p(ud), p(out["bi"]),
+                                     ctypes.c_void_p(st))
+    check(rc, "hp3d_gpu_dirichlet_lift")
+
+
+def pack_element_vectors(batch, vec):
+    """(nel, width) per-element interface vectors (caller order) -> a packed
+    device vector of batch.size_v slots (internal order, 0 beyond ni)."""
+    import torch
+
+    v = np.zeros(batch.size_v)
+    ni = batch._ni
+    e_v = np.repeat(np.arange(batch.nel), ni)
+    li = np.arange(int(ni.sum())) - np.repeat(np.cumsum(ni) - ni, ni)
+    v[batch.off_v_h[e_v] + li] = np.asarray(vec)[batch.perm[e_v], li]
+    return torch.as_tensor(v, device=batch.device)
+
+
 def solve_problem(mesh, space=H1, load=None, exact=None, precision=0, dirichlet=True,
-                  solver="direct", rtol=1e-12):
+                  solver="direct", rtol=1e-12, boundary=None, numbering="blocks"):
     """Element matrices, condensation, assembly, solve, back substitution and
-    element errors for a mesh with homogeneous Dirichlet data.
+    element errors for a mesh with Dirichlet data on the whole boundary:
+    homogeneous, or (boundary = a selector of hp3d_amd.dirichlet.exact_h1, or
+    a callable x -> (u, grad u)) the projection-based interpolant of the
+    exact solution on the boundary nodes (update_Ddof), lifted into the
+    condensed loads as celem_systemI does (celem_systemI.F90:794-808).
 
     Returns a dict with the global solution x, the element dof vectors u
     (internal order), the squared error and exact norm sums (errorH, rnormH
@@ -360,13 +447,22 @@ def solve_problem(mesh, space=H1, load=None, exact=None, precision=0, dirichlet=
     batch = ElementBatch.from_mesh(space, mesh, load=load)
     out = batch.run(precision)
     batch.check_status()
-    dm = DofMap(mesh, space, dirichlet)
+    dm = DofMap(mesh, space, dirichlet, numbering=numbering)
+    ud = None
+    if boundary is not None and dirichlet:
+        from .dirichlet import dirichlet_values
+
+        ud = pack_element_vectors(batch, dirichlet_values(mesh, dm, boundary))
+        dirichlet_lift(batch, out, ud)
     asm = Assembly(batch, dm)
     vals = asm.matrix_values(out["aii"])
     b = asm.rhs(out["bi"])
     solver = Solver(asm.csr_host(vals)) if solver == "direct" else PCGSolver(asm, vals, rtol=rtol)
     x = solver(b)
-    u = back_substitute(batch, out, asm.scatter(x))
+    ue = asm.scatter(x)
+    if ud is not None:
+        ue = ue + ud
+    u = back_substitute(batch, out, ue)
     errs, _ = field(batch, FIELD_ERROR, u, exact=exact)
     torch.cuda.synchronize()
     e = errs.cpu().numpy()

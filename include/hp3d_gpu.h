@@ -67,6 +67,7 @@ extern "C" {
 #define HP3D_LOAD_CONV 2      /* f = -2 sin(pi x) sinh(pi y)                 */
 #define HP3D_LOAD_ONE 3       /* f = 1                                        */
 #define HP3D_LOAD_POLY2 4     /* f = -Laplace(u), u = x(1-x) y(1-y) z(1-z)    */
+#define HP3D_LOAD_XYZ 5       /* f = 0, u = x y z (test/poly_pois.F90 exact)  */
 #define HP3D_LOAD_CURL_SIN 10 /* J = (1 + 2 pi^2) sin(pi y) sin(pi z) e_x     */
 
 /* ------------------------------------------------------------ lifecycle */
@@ -160,8 +161,9 @@ size_t hp3d_gpu_elem_workspace(int space, int precision, int nel, int max_ndof);
  *   bi   at off_v[e]: condensed load (may alias bin)
  *   aschur at off_s[e] (nb_e x ni_e, leading dimension nb_e), bschur at off_v[e]
  *   workspace: device memory of hp3d_gpu_stc_workspace(nel, max_ndof,
- *   max_nbubble) bytes (one slab per element: the bubble factor and the
- *   forward-substituted interface columns)
+ *   max_nbubble) bytes (one slab per element: the bubble block's Cholesky
+ *   factor in 16 x 16 tiles, its transformed copy and the inverted diagonal
+ *   blocks; independent of max_ndof)
  *   max_ndof, max_nbubble bound N_e and nb_e over the batch (an element above
  *   them reports HP3D_E_ARG in status[e])
  *   status[e]: 0, a negative code, or 1 when status[e] already held 1 (the
@@ -235,6 +237,29 @@ int hp3d_gpu_celem_batch_noalloc(int space, int precision, int nel, const int *n
 int hp3d_gpu_stc_bwd_batch(int space, int nel, const int *norder, const long long *off_s,
                            const double *aschur, const long long *off_v, const double *bschur,
                            double *x, void *stream);
+
+/*
+ * Dirichlet lift of condensed element loads (celem_systemI.F90:794-808:
+ * ZBMOD(:) -= ZAMOD(:,k2) ZDOFD(k2) for the Dirichlet dofs k2), device
+ * pointers: bi[off_v[e] + k] -= sum_k2 Aii_e(k, k2) ud[off_v[e] + k2] over the
+ * interface dofs (Aii_e at off_c[e], leading dimension ni_e), where ud holds
+ * the element's Dirichlet dof values and 0 at its free dofs.
+ */
+int hp3d_gpu_dirichlet_lift(int space, int nel, const int *norder, const long long *off_c,
+                            const double *aii, const long long *off_v, const double *ud, double *bi,
+                            void *stream);
+
+/*
+ * Shape functions of one hexahedron at npts master-element points xi
+ * (3 x npts, in [0,1]^3), HOST memory and host arithmetic, from the same dof
+ * descriptors and 1D factor formulas as the kernels: shape3DH
+ * (ContExactSequence.F90:397-437) gives val[p*N + k] and the reference
+ * gradient grad[3*(p*N + k) + c]; shape3DE (:468-508) gives the vector
+ * val[3*(p*N + k) + c] and the reference curl in grad.  For set-up work on
+ * the host (the Dirichlet data of update_Ddof).
+ */
+int hp3d_gpu_shape(int space, const int norder[19], const int nori_e[12], const int nori_f[6],
+                   int npts, const double *xi, double *val, double *grad);
 
 /* ------------------------------------------------ solution-level entries */
 

@@ -24,6 +24,7 @@ MAX_NINT3 = 392
 
 LOAD_ZERO, LOAD_SINSINSIN, LOAD_CONV, LOAD_ONE, LOAD_CURL_SIN = 0, 1, 2, 3, 10
 LOAD_POLY2 = 4
+LOAD_XYZ = 5
 
 _i = ctypes.c_int
 _d = ctypes.c_double

@@ -1286,6 +1286,12 @@ void orc_exact_h1(int sel, const double *x, double *u, double *du)
         du[2] = a * b * (1.0 - 2.0 * x[2]);
         break;
     }
+    case ORC_LOAD_XYZ: /* test/poly_pois.F90 exact: u = x y z */
+        *u = x[0] * x[1] * x[2];
+        du[0] = x[1] * x[2];
+        du[1] = x[0] * x[2];
+        du[2] = x[0] * x[1];
+        break;
     default:
         break;
     }

@@ -33,6 +33,7 @@ extern "C" {
 #define ORC_LOAD_CONV 2      /* f = -2 sin(pi x) sinh(pi y)  (u = sin(pi x) sinh(pi y) z^2) */
 #define ORC_LOAD_ONE 3       /* f = 1 */
 #define ORC_LOAD_POLY2 4     /* f = -Laplace(u), u = x(1-x) y(1-y) z(1-z) */
+#define ORC_LOAD_XYZ 5       /* f = 0, u = x y z */
 #define ORC_LOAD_CURL_SIN 10 /* J = (1+2 pi^2) sin(pi y) sin(pi z) e_x */
 
 void orc_decod(int nick, int mod, int n, int *narr);

@@ -78,9 +78,14 @@ def test_entries_before_init_and_bad_arguments():
     assert lib.hp3d_gpu_elem_batch(0, 0, 0, None, None, None, None, 8, None, 0, None, 0, None,
                                    None, None, None, None, 1, None) == 0  # empty batch
     assert lib.hp3d_gpu_stc_workspace(10, 125, 0) == 0
-    # slab per element: nb x nb factor + nb x (ni + 1) columns, both rounded up to 16
-    assert lib.hp3d_gpu_stc_workspace(10, 125, 27) == 10 * (32 * 32 + 32 * 128) * 8
-    assert lib.hp3d_gpu_stc_workspace(10, 540, 240) == 10 * (240 * 240 + 240 * 544) * 8
+    # slab per element, in 16 x 16 tiles over nbk = ceil(nb / 16) block rows: the
+    # lower factor L and its transformed copy (nbk (nbk + 1) / 2 tiles each) and
+    # the inverted diagonal blocks and their transposes (nbk each); independent
+    # of N
+    for nb in (27, 240):
+        nbk = -(-nb // 16)
+        assert lib.hp3d_gpu_stc_workspace(10, 540, nb) == 10 * (nbk * (nbk + 1) + 2 * nbk) * 256 * 8
+    assert lib.hp3d_gpu_stc_workspace(10, 125, 27) == lib.hp3d_gpu_stc_workspace(10, 540, 27)
     # geometry slabs: one per element of the block-tiled path (343 points x 10 or
     # 21 doubles + a flag), none for the single-block kernel
     assert lib.hp3d_gpu_elem_workspace(0, 1, 10, 128) == 0

@@ -94,3 +94,67 @@ def test_fortran_elem_and_celem_on_gpu(gpu, space):
     asc = parts[2].reshape(ni, nb).T
     for got, ref in ((aii, aii_r), (parts[1], bi_r), (asc, as_r), (parts[3], bs_r)):
         assert np.linalg.norm(got - ref) <= 1e-11 * np.linalg.norm(ref)
+
+
+def batch_input(mode, space, m, load):
+    xn = m.xnod()
+    lines = ["%d" % mode, "%d 0 %d 8 %d" % (space, m.nel, load)]
+    for e in range(m.nel):
+        lines.append(" ".join(str(int(v)) for v in list(m.norder[e]) + list(m.nori_e[e])
+                              + list(m.nori_f[e])))
+        lines.append(" ".join("%.17e" % v for v in xn[e].reshape(-1)))
+        lines.append("%.17e" % m.coef[e])
+    return "\n".join(lines) + "\n"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("space", [0, 1])
+@pytest.mark.parametrize("mode", [4, 5])
+def test_fortran_batched_celem_on_gpu(gpu, space, mode):
+    """A Fortran element loop over a batch: codes, geometry and offsets packed
+    on the host, staged with hipMalloc/hipMemcpy, condensed by
+    hp3d_gpu_celem_batch (mode 4) or hp3d_gpu_celem_batch_noalloc (mode 5);
+    mixed orders up to p = 5 in H1 and p = 3 in H(curl), so that both the
+    fused and the two-pass condensation serve the batch."""
+    rng = np.random.default_rng(20 + space)
+    po = rng.integers(1, 6 if space == 0 else 4, size=(4, 3))
+    po[0] = 5 if space == 0 else 3
+    m = mesh.structured((2, 2, 1), porder=po, perturb=0.1, mesh_seed=3, perm_seed=4,
+                        coef=np.exp(rng.normal(0, 0.5, 4)))
+    load = oracle.LOAD_SINSINSIN if space == 0 else oracle.LOAD_CURL_SIN
+    r = run_driver(batch_input(mode, space, m, load))
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr
+    tok = r.stdout.split()
+    assert tok[:2] == ["batch", str(m.nel)]
+    assert [int(t) for t in tok[2:2 + m.nel]] == [0] * m.nel
+    vals = np.array(tok[2 + m.nel:], float)
+    cnt = oracle.ndof_h1 if space == 0 else oracle.ndof_hcurl
+    bub = oracle.nbubble_h1 if space == 0 else oracle.nbubble_hcurl
+    ns = [int(cnt(m.norder[e])) for e in range(m.nel)]
+    nbs = [int(bub(m.norder[e][18])) for e in range(m.nel)]
+    sc = sum((n - nb) ** 2 for n, nb in zip(ns, nbs))
+    sv = sum(ns)
+    ss = sum(nb * (n - nb) for n, nb in zip(ns, nbs))
+    assert vals.size == sc + 2 * sv + ss
+    aii_all, bi_all = vals[:sc], vals[sc:sc + sv]
+    bs_all, as_all = vals[sc + sv:sc + 2 * sv], vals[sc + 2 * sv:]
+    xn = m.xnod()
+    oc = ov = os_ = 0
+    for e in range(m.nel):
+        n, nb = ns[e], nbs[e]
+        ni = n - nb
+        no, ne, nf = m.norder[e], m.nori_e[e], m.nori_f[e]
+        if space == 0:
+            k, b, _ = oracle.elem_h1(no, ne, nf, xn[e].T, kappa=m.coef[e], load=load)
+        else:
+            k, b, _ = oracle.elem_hcurl(no, ne, nf, xn[e].T, eps=m.coef[e], load=load)
+        aii_r, bi_r, as_r, bs_r = oracle.stc_fwd(k, b, nb, herm=True)
+        aii = aii_all[oc:oc + ni * ni].reshape(ni, ni).T
+        asc = as_all[os_:os_ + nb * ni].reshape(ni, nb).T
+        got = ((aii, aii_r), (bi_all[ov:ov + ni], bi_r), (asc, as_r), (bs_all[ov:ov + nb], bs_r))
+        for g, ref in got:
+            if ref.size:
+                assert np.linalg.norm(g - ref) <= 1e-11 * np.linalg.norm(ref), (e, n, nb)
+        oc += ni * ni
+        ov += n
+        os_ += nb * ni

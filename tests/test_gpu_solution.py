@@ -67,9 +67,14 @@ def test_solution_matches_cpu_reference(gpu, space, p, load):
 
     coef = np.exp(np.random.default_rng(7).normal(0, 0.5, 27))
     m = mesh.structured((3, 3, 3), porder=p, perturb=0.1, mesh_seed=6, perm_seed=8, coef=coef)
-    g = solve_problem(m, space, load)
+    # the CPU reference numbers the dofs on first visit, as mumps_sc does
+    g = solve_problem(m, space, load, numbering="first_visit")
     c = sr.solve(m, space, load)
     assert rel(g["x"].cpu().numpy(), c["x"]) < 1e-11
+    # the block numbering permutes the same system: the same element vectors
+    gb = solve_problem(m, space, load)
+    for e in range(m.nel):
+        assert rel(element_vec(gb["batch"], gb["u"], e), c["u"][e]) < 1e-10
     for e in range(m.nel):
         assert rel(element_vec(g["batch"], g["u"], e), c["u"][e]) < 1e-10
     assert abs(g["error2"] - c["error2"]) < 1e-10 * c["error2"]
@@ -133,3 +138,103 @@ def test_adjoint_gradient(gpu, precision):
         tm[e] -= h
         fd = (p64.objective(tp)[0] - p64.objective(tm)[0]) / (2 * h)
         assert abs(fd - g[e]) <= (1e-6 if precision == 0 else 1e-5) * np.abs(g).max(), (e, fd, g[e])
+
+
+def test_poly_pois_on_gpu(gpu):
+    """test/poly_pois.F90:60-101 as it states it: p = 1 on the unit cube of
+    files/mesh/hexa_orient_0 after one and two global refinements, Dirichlet
+    data u = xyz on the whole boundary (projection-based interpolation), f = 0,
+    H1 error <= 1e-15; and the first-visit numbering gives the CPU
+    reference's global solution."""
+    from hp3d_amd.solve import solve_problem
+
+    for nref in (1, 2):
+        n = 2 ** nref
+        m = mesh.structured((n, n, n), porder=1, perm_seed=nref)
+        g = solve_problem(m, 0, oracle.LOAD_ZERO, exact=oracle.LOAD_XYZ, boundary=oracle.LOAD_XYZ,
+                          numbering="first_visit")
+        assert np.sqrt(g["error2"]) <= 1e-15, (nref, np.sqrt(g["error2"]))
+        if nref == 2:
+            from hp3d_amd import dirichlet
+            from hp3d_amd.solve import DofMap
+
+            ud = dirichlet.dirichlet_values(m, DofMap(m, 0), oracle.LOAD_XYZ)
+            c = sr.solve(m, 0, oracle.LOAD_ZERO, exact=oracle.LOAD_XYZ, ud=ud)
+            assert rel(g["x"].cpu().numpy(), c["x"]) < 1e-13
+
+
+def test_conv_pois_on_gpu(gpu):
+    """test/conv_pois.F90:35-75 as it states it: p = 2, u = sin(pi x) sinh(pi y)
+    z^2 with its Dirichlet data, three refinements (2^3, 4^3, 8^3), rate in
+    [1.95, 2.05]; each level's global solution and error against the CPU
+    reference (first-visit numbering on both sides)."""
+    from hp3d_amd import dirichlet
+    from hp3d_amd.solve import DofMap, solve_problem
+
+    errs = []
+    for nref in (1, 2, 3):
+        n = 2 ** nref
+        m = mesh.structured((n, n, n), porder=2, perm_seed=nref)
+        g = solve_problem(m, 0, oracle.LOAD_CONV, boundary=oracle.LOAD_CONV, numbering="first_visit")
+        errs.append(np.sqrt(g["error2"]))
+        if nref < 3:
+            ud = dirichlet.dirichlet_values(m, DofMap(m, 0), oracle.LOAD_CONV)
+            c = sr.solve(m, 0, oracle.LOAD_CONV, ud=ud)
+            assert rel(g["x"].cpu().numpy(), c["x"]) < 1e-11
+            assert abs(g["error2"] - c["error2"]) < 1e-10 * c["error2"]
+    for i in (1, 2):
+        rate = np.log(errs[i - 1] / errs[i]) / np.log(2.0)
+        assert 1.95 <= rate <= 2.05, (errs, rate)
+
+
+def test_dirichlet_lift_mixed_orders_distorted(gpu):
+    """Non-zero Dirichlet data on a distorted mixed-order mesh with variable
+    coefficients: the device lift (hp3d_gpu_dirichlet_lift), assembly, solve
+    and back substitution with ud against the CPU reference with the same
+    boundary values, element by element."""
+    from hp3d_amd import dirichlet
+    from hp3d_amd.solve import DofMap, solve_problem
+
+    rng = np.random.default_rng(11)
+    po = rng.integers(1, 5, size=(3 * 3 * 2, 3))
+    m = mesh.structured((3, 3, 2), porder=po, perturb=0.1, mesh_seed=2, perm_seed=6,
+                        coef=np.exp(rng.normal(0, 0.5, 18)))
+    g = solve_problem(m, 0, oracle.LOAD_SINSINSIN, boundary=oracle.LOAD_CONV, exact=oracle.LOAD_CONV,
+                      numbering="first_visit")
+    ud = dirichlet.dirichlet_values(m, DofMap(m, 0), oracle.LOAD_CONV)
+    c = sr.solve(m, 0, oracle.LOAD_SINSINSIN, exact=oracle.LOAD_CONV, ud=ud)
+    assert rel(g["x"].cpu().numpy(), c["x"]) < 1e-11
+    for e in range(m.nel):
+        assert rel(element_vec(g["batch"], g["u"], e), c["u"][e]) < 1e-10
+    assert abs(g["error2"] - c["error2"]) < 1e-10 * c["error2"]
+
+
+def test_hcurl_p5_solution_fp32_against_fp64_oracle(gpu):
+    """The precision workload's solution form (H(curl) at uniform p = 5, +-0.1h
+    vertex perturbation, eps_e ~ U[1, 4], J = curl curl E* + E*) on a 4^3 mesh
+    of the same recipe: the GPU FP64 solution equals the FP64 oracle's, and the
+    FP32-contraction solution differs from it, in the energy norm of the
+    condensed system, by no more than the FP32 rounding of K carries through
+    (measured 5.9e-6; bound 2e-5); its error against E* matches the oracle's
+    to 1e-5 (measured 3e-7).  The FP64 solution differs by 2e-12."""
+    from hp3d_amd.solve import solve_problem
+
+    eps = np.random.default_rng(2003).uniform(1.0, 4.0, 64)
+    m = mesh.structured((4, 4, 4), porder=5, perturb=0.1, mesh_seed=1003, perm_seed=5003, coef=eps)
+    c = sr.solve(m, 1, oracle.LOAD_CURL_SIN)
+    a, x = c["a"], c["x"]
+    xax = x @ (a @ x)
+    diffs = []
+    for precision in (0, 1):
+        g = solve_problem(m, 1, oracle.LOAD_CURL_SIN, precision=precision, numbering="first_visit")
+        d = g["x"].cpu().numpy() - x
+        diffs.append(np.sqrt(d @ (a @ d) / xax))
+        err_ratio = np.sqrt(g["error2"] / c["error2"])
+        print("precision %d: energy-norm difference %.3e, error ratio - 1 = %.3e"
+              % (precision, diffs[-1], err_ratio - 1))
+        if precision == 0:
+            assert diffs[-1] < 1e-10
+            assert abs(err_ratio - 1) < 1e-10
+        else:
+            assert diffs[-1] < 2e-5
+            assert abs(err_ratio - 1) < 1e-5
