@@ -12,7 +12,8 @@ value      element matrices/s of the batched FP32-contraction form (FP64
            geometry and bubble factorisation), whole job
 forms      the same for the batched FP64 form and for the plain FP64 drop-in
            (one C-ABI call per element from the host, as the reference's
-           OpenMP loop would make it)
+           OpenMP loop would make it, from one thread and from a pool of
+           host threads)
 roofline   the step's launches: algorithmic flops (k N (N+1) per element plus
            the condensation count) over their mean duration (HIP events on
            the launch stream)
@@ -26,6 +27,7 @@ import argparse
 import json
 import os
 import sys
+import threading
 import time
 
 import numpy as np
@@ -95,18 +97,31 @@ def time_steps(batch, out, precision, steps, warmup):
     return (t1 - t0) / steps, elem_ms
 
 
-def plain_dropin_rate(m, space, nsample):
-    """Per-element C-ABI calls (host arrays in and out), as the reference loop would make."""
+def plain_dropin_rate(m, space, nsample, threads=1):
+    """Per-element C-ABI calls (host arrays in and out), as the reference loop
+    would make them: from one thread, or from `threads` host threads at once
+    (the OpenMP loop; each thread has its own context, made before timing)."""
+    from concurrent.futures import ThreadPoolExecutor
+
     import hp3d_amd
 
     xn = m.xnod()
     load = hp3d_amd.LOAD_SINSINSIN if space == 0 else hp3d_amd.LOAD_CURL_SIN
-    hp3d_amd.celem(space, m.norder[0], m.nori_e[0], m.nori_f[0], xn[0], coef=m.coef[0], load=load)
-    t0 = time.perf_counter()
-    for e in range(nsample):
-        hp3d_amd.celem(space, m.norder[e], m.nori_e[e], m.nori_f[e], xn[e], coef=m.coef[e],
-                       load=load)
-    return nsample / (time.perf_counter() - t0)
+    one = lambda e: hp3d_amd.celem(space, m.norder[e], m.nori_e[e], m.nori_f[e], xn[e],
+                                   coef=m.coef[e], load=load)
+    if threads == 1:
+        one(0)
+        t0 = time.perf_counter()
+        for e in range(nsample):
+            one(e)
+        return nsample / (time.perf_counter() - t0)
+    with ThreadPoolExecutor(threads) as ex:
+        # one call per worker first, held at a barrier so each worker makes its context
+        bar = threading.Barrier(threads)
+        list(ex.map(lambda e: (one(e), bar.wait(timeout=300)), range(threads)))
+        t0 = time.perf_counter()
+        list(ex.map(one, range(nsample)))
+        return nsample / (time.perf_counter() - t0)
 
 
 def host_cpu():
@@ -147,8 +162,10 @@ def cpu_baseline(m, batch, out32, out64, seconds=15.0, single_core=True):
     loop = lambda sl, nt: oracle.element_loop_blas(
         space, m.norder[sl], m.nori_e[sl], m.nori_f[sl], m.xnod()[sl], m.coef[sl], load,
         ldk=ldk, nthreads=nt)
-    # calibrate the sample so the timed run is ~seconds of work
-    probe = min(m.nel, 2 * threads)
+    # calibrate the sample so the timed run is ~seconds of work: a first call
+    # warms the BLAS and OpenMP thread pools, a second one gives the rate
+    probe = min(m.nel, max(64, 8 * threads))
+    loop(slice(0, min(m.nel, 2 * threads)), threads)
     t0 = time.perf_counter()
     loop(slice(0, probe), threads)
     dt = max(time.perf_counter() - t0, 1e-3)
@@ -173,8 +190,8 @@ def cpu_baseline(m, batch, out32, out64, seconds=15.0, single_core=True):
                      "DGEMM) per element, OpenMP dynamic schedule, %.1f s" % (ns, m.nel, dt),
            "blas": blas, "cpu_model": model, "cpus_visible": ncpu}
     if single_core:
-        # the same loop on one core, on a smaller sample
-        n1 = int(min(ns, max(1, ns / threads / 3)))
+        # the same loop on one core, on a sample of about a third of `seconds`
+        n1 = int(min(ns, max(1, ns / dt / threads * seconds / 3)))
         t0 = time.perf_counter()
         loop(slice(0, n1), 1)
         dt1 = time.perf_counter() - t0
@@ -317,6 +334,7 @@ def measure(workload, args, headline):
     amax, amed = rel_err_per_element(batch, out32["aii"], out64["aii"], batch._ni * batch._ni)
     nd = min(args.dropin_sample if space == 0 else 16, nel)
     dropin = plain_dropin_rate(m, space, nd)
+    dropin_t = plain_dropin_rate(m, space, 2 * nd, threads=cpu_threads())
     forms = {
         "batched_fp32": {"element_matrices_per_s": value, "ms_per_step": s32 * 1e3,
                          "elem_kernel_ms": e32, "condensed_dof_per_s": nb_total / s32,
@@ -325,6 +343,9 @@ def measure(workload, args, headline):
                          "elem_kernel_ms": e64, "condensed_dof_per_s": nb_total / s64},
         "plain_fp64_dropin": {"element_matrices_per_s": dropin,
                               "note": "hp3d_gpu_celem per element, host arrays, %d elements" % nd},
+        "plain_fp64_dropin_threads": {"element_matrices_per_s": dropin_t, "threads": cpu_threads(),
+                                      "note": "the same calls from an OpenMP-like pool of host "
+                                              "threads, %d elements" % (2 * nd)},
         "fp32_over_dropin": value / dropin,
         "fp32_over_fp64_batched": s64 / s32,
     }

@@ -32,6 +32,7 @@
 #include <algorithm>
 #include <mutex>
 #include <type_traits>
+#include <vector>
 
 #include "../../include/hp3d_gpu.h"
 #include "hp3d_tables.h"
@@ -2308,28 +2309,38 @@ __global__ __launch_bounds__(WG) void csr_spmv_kernel(long long nrows, const lon
 
 // ------------------------------------------------------------ host state
 
-struct Lib {
-    std::mutex mu;
-    int device = -1;
+// Per-element entries: each host thread that calls them gets its own context
+// (stream, device staging, pinned host staging), created on its first call
+// after hp3d_gpu_init, so the calls of an OpenMP element loop overlap on the
+// GPU.  hp3d_gpu_finalize frees every context; a thread's context from before
+// a finalize is recognised by its generation and replaced on its next call.
+struct Ctx {
     hipStream_t stream = nullptr;
-    int max_ndof = 0;
-    int nbm = 0; // largest bubble block the per-element workspace holds
-    // per-element staging (device)
-    int *d_codes = nullptr; // norder[19], nori_e[12], nori_f[6]
+    int *d_codes = nullptr; // norder[19], nori_e[12], nori_f[6]; n, nb for stc_fwd
     double *d_xnod = nullptr, *d_coef = nullptr, *d_k = nullptr, *d_b = nullptr;
     double *d_aii = nullptr, *d_bi = nullptr, *d_as = nullptr, *d_bs = nullptr, *d_ws = nullptr;
     long long *d_off = nullptr; // zeros
     int *d_status = nullptr;
-    // pinned host staging
     double *h_k = nullptr, *h_b = nullptr, *h_as = nullptr, *h_bs = nullptr, *h_x = nullptr;
     int *h_codes = nullptr, *h_status = nullptr;
-    char err[256] = {0};
+};
+
+struct Lib {
+    std::mutex mu;          // init, finalize and context creation
+    int device = -1;
+    int max_ndof = 0;
+    int nbm = 0;            // largest bubble block a context's workspace holds
+    unsigned gen = 1;       // bumped by every finalize
+    std::vector<Ctx *> ctxs;
 };
 static Lib g;
+static thread_local Ctx *t_ctx = nullptr;
+static thread_local unsigned t_gen = 0;
+static thread_local char t_err[256] = {0}; // hp3d_gpu_last_error: per calling thread
 
 static int hip_fail(hipError_t r, const char *what)
 {
-    snprintf(g.err, sizeof(g.err), "%s: %s", what, hipGetErrorString(r));
+    snprintf(t_err, sizeof(t_err), "%s: %s", what, hipGetErrorString(r));
     return HP3D_E_HIP;
 }
 #define HIPCHK(x)                                     \
@@ -2433,7 +2444,7 @@ static int scratch_dev(double **ptr)
     int dev = -1;
     HIPCHK(hipGetDevice(&dev));
     if (dev < 0 || dev >= MAX_DEVICES) {
-        snprintf(g.err, sizeof(g.err), "device %d: no scratch slot", dev);
+        snprintf(t_err, sizeof(t_err), "device %d: no scratch slot", dev);
         return HP3D_E_ARG;
     }
     std::lock_guard<std::mutex> lk(g_scratch_mu);
@@ -2463,7 +2474,7 @@ using namespace hp3d;
 
 extern "C" {
 
-const char *hp3d_gpu_last_error(void) { return g.err; }
+const char *hp3d_gpu_last_error(void) { return t_err; }
 
 int hp3d_gpu_set_option(int option, int value)
 {
@@ -2486,60 +2497,108 @@ int hp3d_gpu_ndof(int space, const int norder[19], int *nrdof, int *nrdofb)
     return 0;
 }
 
-int hp3d_gpu_init(int device, int max_ndof)
+static void ctx_free(Ctx *c)
 {
+    if (c->stream) hipStreamSynchronize(c->stream);
+    void *dev[] = {c->d_codes, c->d_xnod, c->d_coef, c->d_k, c->d_b, c->d_aii, c->d_bi,
+                   c->d_as,    c->d_bs,   c->d_ws,   c->d_off, c->d_status};
+    for (void *p : dev)
+        if (p) hipFree(p);
+    void *host[] = {c->h_k, c->h_b, c->h_as, c->h_bs, c->h_x, c->h_codes, c->h_status};
+    for (void *p : host)
+        if (p) hipHostFree(p);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+// allocate a context on the library's device (caller holds g.mu)
+static int ctx_alloc(Ctx *c)
+{
+    HIPCHK(hipSetDevice(g.device));
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    const int mx = g.max_ndof;
+    const size_t nn = (size_t)mx * mx;
+    HIPCHK(hipMalloc(&c->d_codes, 64 * sizeof(int)));
+    HIPCHK(hipMalloc(&c->d_xnod, 3 * (size_t)MAXN_H1 * sizeof(double)));
+    HIPCHK(hipMalloc(&c->d_coef, sizeof(double)));
+    HIPCHK(hipMalloc(&c->d_k, nn * sizeof(double)));
+    HIPCHK(hipMalloc(&c->d_b, mx * sizeof(double)));
+    HIPCHK(hipMalloc(&c->d_aii, nn * sizeof(double)));
+    HIPCHK(hipMalloc(&c->d_bi, mx * sizeof(double)));
+    HIPCHK(hipMalloc(&c->d_as, nn * sizeof(double)));
+    HIPCHK(hipMalloc(&c->d_bs, mx * sizeof(double)));
+    HIPCHK(hipMalloc(&c->d_ws, std::max(stc_slab(rnd16(g.nbm)), geo_slab(HP3D_HCURL)) *
+                                   sizeof(double)));
+    HIPCHK(hipMalloc(&c->d_off, sizeof(long long)));
+    HIPCHK(hipMalloc(&c->d_status, 2 * sizeof(int)));
+    HIPCHK(hipHostMalloc(&c->h_k, nn * sizeof(double)));
+    HIPCHK(hipHostMalloc(&c->h_b, mx * sizeof(double)));
+    HIPCHK(hipHostMalloc(&c->h_as, nn * sizeof(double)));
+    HIPCHK(hipHostMalloc(&c->h_bs, mx * sizeof(double)));
+    HIPCHK(hipHostMalloc(&c->h_x, (3 * (size_t)MAXN_H1 + 1) * sizeof(double)));
+    HIPCHK(hipHostMalloc(&c->h_codes, 64 * sizeof(int)));
+    HIPCHK(hipHostMalloc(&c->h_status, 2 * sizeof(int)));
+    HIPCHK(hipMemsetAsync(c->d_off, 0, sizeof(long long), c->stream));
+    HIPCHK(hipMemsetAsync(c->d_status, 0, 2 * sizeof(int), c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// the calling thread's context, created on its first call after hp3d_gpu_init
+static int ctx_get(Ctx **out)
+{
+    if (t_ctx && t_gen == g.gen) {
+        *out = t_ctx;
+        return 0;
+    }
     std::lock_guard<std::mutex> lk(g.mu);
-    if (g.device >= 0) return 0;
-    int cnt = 0;
-    if (hipGetDeviceCount(&cnt) != hipSuccess || cnt <= device || device < 0) {
-        snprintf(g.err, sizeof(g.err), "no HIP device %d", device);
+    if (g.device < 0) {
+        snprintf(t_err, sizeof(t_err), "hp3d_gpu_init not called");
         return HP3D_E_INIT;
     }
-    HIPCHK(hipSetDevice(device));
-    double *scr = nullptr;
-    if (int r = scratch_dev(&scr)) return r;
-    if (max_ndof <= 0) max_ndof = MAXN_E;
-    g.max_ndof = max_ndof;
-    HIPCHK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
-    size_t nn = (size_t)max_ndof * max_ndof;
-    HIPCHK(hipMalloc(&g.d_codes, 64 * sizeof(int)));
-    HIPCHK(hipMalloc(&g.d_xnod, 3 * (size_t)MAXN_H1 * sizeof(double)));
-    HIPCHK(hipMalloc(&g.d_coef, sizeof(double)));
-    HIPCHK(hipMalloc(&g.d_k, nn * sizeof(double)));
-    HIPCHK(hipMalloc(&g.d_b, max_ndof * sizeof(double)));
-    HIPCHK(hipMalloc(&g.d_aii, nn * sizeof(double)));
-    HIPCHK(hipMalloc(&g.d_bi, max_ndof * sizeof(double)));
-    HIPCHK(hipMalloc(&g.d_as, nn * sizeof(double)));
-    HIPCHK(hipMalloc(&g.d_bs, max_ndof * sizeof(double)));
-    g.nbm = max_bubbles(HP3D_HCURL, max_ndof);
-    HIPCHK(hipMalloc(&g.d_ws, std::max(stc_slab(rnd16(g.nbm)), geo_slab(HP3D_HCURL)) *
-                                  sizeof(double)));
-    HIPCHK(hipMalloc(&g.d_off, sizeof(long long)));
-    HIPCHK(hipMemset(g.d_off, 0, sizeof(long long)));
-    HIPCHK(hipMalloc(&g.d_status, 2 * sizeof(int)));
-    HIPCHK(hipMemset(g.d_status, 0, 2 * sizeof(int)));
-    HIPCHK(hipHostMalloc(&g.h_k, nn * sizeof(double)));
-    HIPCHK(hipHostMalloc(&g.h_b, max_ndof * sizeof(double)));
-    HIPCHK(hipHostMalloc(&g.h_as, nn * sizeof(double)));
-    HIPCHK(hipHostMalloc(&g.h_bs, max_ndof * sizeof(double)));
-    HIPCHK(hipHostMalloc(&g.h_x, (3 * (size_t)MAXN_H1 + 1) * sizeof(double)));
-    HIPCHK(hipHostMalloc(&g.h_codes, 64 * sizeof(int)));
-    HIPCHK(hipHostMalloc(&g.h_status, 2 * sizeof(int)));
-    g.device = device;
+    Ctx *c = new Ctx;
+    if (int r = ctx_alloc(c)) {
+        ctx_free(c);
+        return r;
+    }
+    g.ctxs.push_back(c);
+    t_ctx = c;
+    t_gen = g.gen;
+    *out = c;
     return 0;
+}
+
+int hp3d_gpu_init(int device, int max_ndof)
+{
+    {
+        std::lock_guard<std::mutex> lk(g.mu);
+        if (g.device < 0) {
+            int cnt = 0;
+            if (hipGetDeviceCount(&cnt) != hipSuccess || cnt <= device || device < 0) {
+                snprintf(t_err, sizeof(t_err), "no HIP device %d", device);
+                return HP3D_E_INIT;
+            }
+            HIPCHK(hipSetDevice(device));
+            double *scr = nullptr;
+            if (int r = scratch_dev(&scr)) return r;
+            if (max_ndof <= 0) max_ndof = MAXN_E;
+            g.max_ndof = max_ndof;
+            g.nbm = max_bubbles(HP3D_HCURL, max_ndof);
+            g.device = device;
+        }
+    }
+    // the calling thread's context now, so that allocation failures show here
+    Ctx *c;
+    return ctx_get(&c);
 }
 
 int hp3d_gpu_finalize(void)
 {
     std::lock_guard<std::mutex> lk(g.mu);
     if (g.device < 0) return 0;
-    hipStreamSynchronize(g.stream);
-    void *dev[] = {g.d_codes, g.d_xnod, g.d_coef, g.d_k, g.d_b, g.d_aii, g.d_bi,
-                   g.d_as,    g.d_bs,   g.d_ws,   g.d_off, g.d_status};
-    for (void *p : dev) hipFree(p);
-    void *host[] = {g.h_k, g.h_b, g.h_as, g.h_bs, g.h_x, g.h_codes, g.h_status};
-    for (void *p : host) hipHostFree(p);
-    hipStreamDestroy(g.stream);
+    for (Ctx *c : g.ctxs) ctx_free(c);
+    g.ctxs.clear();
+    ++g.gen;
     g.device = -1;
     return 0;
 }
@@ -2717,7 +2776,7 @@ int hp3d_gpu_celem_batch_noalloc(int space, int precision, int nel, const int *n
                                     load, off_k, ld_k, kout, off_v, bout, off_c, aii, bi, off_s,
                                     aschur, bschur, nullptr, status, max_ndof, max_nbubble, stream);
     if (per > SCRATCH_BYTES) {
-        snprintf(g.err, sizeof(g.err),
+        snprintf(t_err, sizeof(t_err),
                  "celem_batch_noalloc: %zu bytes of workspace per element (max_ndof %d, "
                  "max_nbubble %d) exceed the %zu-byte scratch",
                  per, max_ndof, max_nbubble, SCRATCH_BYTES);
@@ -2856,13 +2915,9 @@ int hp3d_gpu_csr_spmv(long long nrows, const long long *indptr, const int *indic
 
 // ---------------------------------------------------- per-element drop-in
 
-static int elem_one(int space, const int norder[19], const int nori_e[12], const int nori_f[6],
+static int elem_one(Ctx &c, int space, const int norder[19], const int nori_e[12], const int nori_f[6],
                     const double *xnod, int ngeo, double coef, int load, int &n, int &nb)
 {
-    if (g.device < 0) {
-        snprintf(g.err, sizeof(g.err), "hp3d_gpu_init not called");
-        return HP3D_E_INIT;
-    }
     if (!norder || !nori_e || !nori_f || !xnod || ngeo < 1 || ngeo > MAXN_H1) return HP3D_E_ARG;
     ElemOrders o;
     if (!decode_orders(norder, nori_e, nori_f, o)) return HP3D_E_ORDER;
@@ -2870,28 +2925,28 @@ static int elem_one(int space, const int norder[19], const int nori_e[12], const
     int nh, nbh;
     dof_counts(HP3D_H1, o, nh, nbh);
     if (n > g.max_ndof || ngeo > nh) return HP3D_E_ARG;
-    memcpy(g.h_codes, norder, 19 * sizeof(int));
-    memcpy(g.h_codes + 19, nori_e, 12 * sizeof(int));
-    memcpy(g.h_codes + 31, nori_f, 6 * sizeof(int));
-    memcpy(g.h_x, xnod, 3 * (size_t)ngeo * sizeof(double));
-    g.h_x[3 * ngeo] = coef;
-    HIPCHK(hipMemcpyAsync(g.d_codes, g.h_codes, 37 * sizeof(int), hipMemcpyHostToDevice, g.stream));
-    HIPCHK(hipMemcpyAsync(g.d_xnod, g.h_x, 3 * (size_t)ngeo * sizeof(double), hipMemcpyHostToDevice, g.stream));
-    HIPCHK(hipMemcpyAsync(g.d_coef, g.h_x + 3 * ngeo, sizeof(double), hipMemcpyHostToDevice, g.stream));
-    int r = hp3d_gpu_elem_batch(space, HP3D_FP64, 1, g.d_codes, g.d_codes + 19, g.d_codes + 31,
-                                g.d_xnod, ngeo, g.d_coef, load, g.d_off, 0, g.d_k, g.d_off, g.d_b,
-                                g.d_ws, g.d_status, n, g.stream);
+    memcpy(c.h_codes, norder, 19 * sizeof(int));
+    memcpy(c.h_codes + 19, nori_e, 12 * sizeof(int));
+    memcpy(c.h_codes + 31, nori_f, 6 * sizeof(int));
+    memcpy(c.h_x, xnod, 3 * (size_t)ngeo * sizeof(double));
+    c.h_x[3 * ngeo] = coef;
+    HIPCHK(hipMemcpyAsync(c.d_codes, c.h_codes, 37 * sizeof(int), hipMemcpyHostToDevice, c.stream));
+    HIPCHK(hipMemcpyAsync(c.d_xnod, c.h_x, 3 * (size_t)ngeo * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    HIPCHK(hipMemcpyAsync(c.d_coef, c.h_x + 3 * ngeo, sizeof(double), hipMemcpyHostToDevice, c.stream));
+    int r = hp3d_gpu_elem_batch(space, HP3D_FP64, 1, c.d_codes, c.d_codes + 19, c.d_codes + 31,
+                                c.d_xnod, ngeo, c.d_coef, load, c.d_off, 0, c.d_k, c.d_off, c.d_b,
+                                c.d_ws, c.d_status, n, c.stream);
     return r;
 }
 
-static int copy_back_matrix(double *dst, int ld, const double *src_dev, int rows, int cols,
+static int copy_back_matrix(Ctx &c, double *dst, int ld, const double *src_dev, int rows, int cols,
                             int ld_src)
 {
-    HIPCHK(hipMemcpyAsync(g.h_k, src_dev, (size_t)ld_src * cols * sizeof(double),
-                          hipMemcpyDeviceToHost, g.stream));
-    HIPCHK(hipStreamSynchronize(g.stream));
+    HIPCHK(hipMemcpyAsync(c.h_k, src_dev, (size_t)ld_src * cols * sizeof(double),
+                          hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipStreamSynchronize(c.stream));
     for (int j = 0; j < cols; ++j)
-        memcpy(dst + (size_t)j * ld, g.h_k + (size_t)j * ld_src, rows * sizeof(double));
+        memcpy(dst + (size_t)j * ld, c.h_k + (size_t)j * ld_src, rows * sizeof(double));
     return 0;
 }
 
@@ -2900,19 +2955,21 @@ int hp3d_gpu_elem(int space, const int norder[19], const int nori_e[12], const i
                   int ld_aloc, double *bloc)
 {
     if (space != HP3D_H1 && space != HP3D_HCURL) return HP3D_E_ARG;
-    std::lock_guard<std::mutex> lk(g.mu);
+    Ctx *cp;
+    if (int rc = ctx_get(&cp)) return rc;
+    Ctx &c = *cp;
     int n, nb;
-    int r = elem_one(space, norder, nori_e, nori_f, xnod, ngeo, coef, load, n, nb);
+    int r = elem_one(c, space, norder, nori_e, nori_f, xnod, ngeo, coef, load, n, nb);
     if (r) return r;
     if (!aloc || !bloc || ld_aloc < n) return HP3D_E_ARG;
-    HIPCHK(hipMemcpyAsync(g.h_b, g.d_b, n * sizeof(double), hipMemcpyDeviceToHost, g.stream));
-    HIPCHK(hipMemcpyAsync(g.h_status, g.d_status, sizeof(int), hipMemcpyDeviceToHost, g.stream));
-    if ((r = copy_back_matrix(aloc, ld_aloc, g.d_k, n, n, n))) return r;
-    memcpy(bloc, g.h_b, n * sizeof(double));
-    return *g.h_status;
+    HIPCHK(hipMemcpyAsync(c.h_b, c.d_b, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipMemcpyAsync(c.h_status, c.d_status, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    if ((r = copy_back_matrix(c, aloc, ld_aloc, c.d_k, n, n, n))) return r;
+    memcpy(bloc, c.h_b, n * sizeof(double));
+    return *c.h_status;
 }
 
-static int stc_run(int space, int n, int nb, const int *d_norder, const int *d_nsz,
+static int stc_run(Ctx &c, int space, int n, int nb, const int *d_norder, const int *d_nsz,
                    const double *d_k, const double *d_b, int *d_status)
 {
     StcArgs a;
@@ -2925,66 +2982,64 @@ static int stc_run(int space, int n, int nb, const int *d_norder, const int *d_n
     a.nb_max = g.nbm;
     a.norder = d_norder;
     a.nsz = d_nsz;
-    a.off_k = a.off_v = a.off_c = a.off_s = g.d_off;
+    a.off_k = a.off_v = a.off_c = a.off_s = c.d_off;
     a.kin = d_k;
     a.bin = d_b;
-    a.aii = g.d_aii;
-    a.bi = g.d_bi;
-    a.aschur = g.d_as;
-    a.bschur = g.d_bs;
-    a.ws = g.d_ws;
+    a.aii = c.d_aii;
+    a.bi = c.d_bi;
+    a.aschur = c.d_as;
+    a.bschur = c.d_bs;
+    a.ws = c.d_ws;
     a.status = d_status;
     (void)n;
     // the kernel keeps a negative-Jacobian warning it finds in its status
     // slot: start from 0 so nothing of an earlier call survives
-    HIPCHK(hipMemsetAsync(d_status, 0, sizeof(int), g.stream));
-    launch_stc<double>(a, nb, g.stream);
+    HIPCHK(hipMemsetAsync(d_status, 0, sizeof(int), c.stream));
+    launch_stc<double>(a, nb, c.stream);
     HIPCHK(hipGetLastError());
     return 0;
 }
 
 // copy the condensed system back into the caller's ALOC/BLOC and CLOC arrays
-static int stc_copy_back(int ni, int nb, double *aloc, int ld_aloc, double *bloc, double *aschur,
+static int stc_copy_back(Ctx &c, int ni, int nb, double *aloc, int ld_aloc, double *bloc, double *aschur,
                          double *bschur, int &st)
 {
     int r;
-    HIPCHK(hipMemcpyAsync(g.h_status + 1, g.d_status + 1, sizeof(int), hipMemcpyDeviceToHost, g.stream));
-    HIPCHK(hipMemcpyAsync(g.h_b, g.d_bi, ni * sizeof(double), hipMemcpyDeviceToHost, g.stream));
-    HIPCHK(hipMemcpyAsync(g.h_bs, g.d_bs, nb * sizeof(double), hipMemcpyDeviceToHost, g.stream));
-    HIPCHK(hipMemcpyAsync(g.h_as, g.d_as, (size_t)nb * ni * sizeof(double), hipMemcpyDeviceToHost, g.stream));
-    if ((r = copy_back_matrix(aloc, ld_aloc, g.d_aii, ni, ni, ni))) return r;
-    memcpy(bloc, g.h_b, ni * sizeof(double));
-    if (aschur) memcpy(aschur, g.h_as, (size_t)nb * ni * sizeof(double));
-    if (bschur) memcpy(bschur, g.h_bs, nb * sizeof(double));
-    st = g.h_status[1];
+    HIPCHK(hipMemcpyAsync(c.h_status + 1, c.d_status + 1, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipMemcpyAsync(c.h_b, c.d_bi, ni * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipMemcpyAsync(c.h_bs, c.d_bs, nb * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipMemcpyAsync(c.h_as, c.d_as, (size_t)nb * ni * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    if ((r = copy_back_matrix(c, aloc, ld_aloc, c.d_aii, ni, ni, ni))) return r;
+    memcpy(bloc, c.h_b, ni * sizeof(double));
+    if (aschur) memcpy(aschur, c.h_as, (size_t)nb * ni * sizeof(double));
+    if (bschur) memcpy(bschur, c.h_bs, nb * sizeof(double));
+    st = c.h_status[1];
     return 0;
 }
 
 int hp3d_gpu_stc_fwd(int ni, int nb, double *aloc, int ld_aloc, double *bloc, double *aschur,
                      double *bschur)
 {
-    std::lock_guard<std::mutex> lk(g.mu);
-    if (g.device < 0) {
-        snprintf(g.err, sizeof(g.err), "hp3d_gpu_init not called");
-        return HP3D_E_INIT;
-    }
+    Ctx *cp;
+    if (int rc = ctx_get(&cp)) return rc;
+    Ctx &c = *cp;
     int n = ni + nb;
     if (ni < 0 || nb < 0 || n > g.max_ndof || !aloc || !bloc || ld_aloc < n) return HP3D_E_ARG;
     if (nb == 0) return 0;
     if (ni == 0) return HP3D_E_NOIFACE;
     if (nb > max_bubbles(HP3D_HCURL, g.max_ndof)) return HP3D_E_ARG;
     // stage ALOC (ld_aloc) and BLOC into packed device buffers
-    for (int j = 0; j < n; ++j) memcpy(g.h_k + (size_t)j * n, aloc + (size_t)j * ld_aloc, n * sizeof(double));
-    memcpy(g.h_b, bloc, n * sizeof(double));
-    g.h_codes[0] = n;
-    g.h_codes[1] = nb;
-    HIPCHK(hipMemcpyAsync(g.d_k, g.h_k, (size_t)n * n * sizeof(double), hipMemcpyHostToDevice, g.stream));
-    HIPCHK(hipMemcpyAsync(g.d_b, g.h_b, n * sizeof(double), hipMemcpyHostToDevice, g.stream));
-    HIPCHK(hipMemcpyAsync(g.d_codes, g.h_codes, 2 * sizeof(int), hipMemcpyHostToDevice, g.stream));
-    int r = stc_run(HP3D_H1, n, nb, nullptr, g.d_codes, g.d_k, g.d_b, g.d_status + 1);
+    for (int j = 0; j < n; ++j) memcpy(c.h_k + (size_t)j * n, aloc + (size_t)j * ld_aloc, n * sizeof(double));
+    memcpy(c.h_b, bloc, n * sizeof(double));
+    c.h_codes[0] = n;
+    c.h_codes[1] = nb;
+    HIPCHK(hipMemcpyAsync(c.d_k, c.h_k, (size_t)n * n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    HIPCHK(hipMemcpyAsync(c.d_b, c.h_b, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    HIPCHK(hipMemcpyAsync(c.d_codes, c.h_codes, 2 * sizeof(int), hipMemcpyHostToDevice, c.stream));
+    int r = stc_run(c, HP3D_H1, n, nb, nullptr, c.d_codes, c.d_k, c.d_b, c.d_status + 1);
     if (r) return r;
     int st;
-    if ((r = stc_copy_back(ni, nb, aloc, ld_aloc, bloc, aschur, bschur, st))) return r;
+    if ((r = stc_copy_back(c, ni, nb, aloc, ld_aloc, bloc, aschur, bschur, st))) return r;
     return st;
 }
 
@@ -2993,24 +3048,26 @@ int hp3d_gpu_celem(int space, const int norder[19], const int nori_e[12], const 
                    int ld_aloc, double *bloc, double *aschur, double *bschur)
 {
     if (space != HP3D_H1 && space != HP3D_HCURL) return HP3D_E_ARG;
-    std::lock_guard<std::mutex> lk(g.mu);
+    Ctx *cp;
+    if (int rc = ctx_get(&cp)) return rc;
+    Ctx &c = *cp;
     int n, nb;
-    int r = elem_one(space, norder, nori_e, nori_f, xnod, ngeo, coef, load, n, nb);
+    int r = elem_one(c, space, norder, nori_e, nori_f, xnod, ngeo, coef, load, n, nb);
     if (r) return r;
     if (!aloc || !bloc || ld_aloc < n) return HP3D_E_ARG;
     int ni = n - nb;
-    HIPCHK(hipMemcpyAsync(g.h_status, g.d_status, sizeof(int), hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipMemcpyAsync(c.h_status, c.d_status, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     if (nb == 0) {
-        HIPCHK(hipMemcpyAsync(g.h_b, g.d_b, n * sizeof(double), hipMemcpyDeviceToHost, g.stream));
-        if ((r = copy_back_matrix(aloc, ld_aloc, g.d_k, n, n, n))) return r;
-        memcpy(bloc, g.h_b, n * sizeof(double));
-        return g.h_status[0];
+        HIPCHK(hipMemcpyAsync(c.h_b, c.d_b, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        if ((r = copy_back_matrix(c, aloc, ld_aloc, c.d_k, n, n, n))) return r;
+        memcpy(bloc, c.h_b, n * sizeof(double));
+        return c.h_status[0];
     }
     if (ni == 0) return HP3D_E_NOIFACE;
-    if ((r = stc_run(space, n, nb, g.d_codes, nullptr, g.d_k, g.d_b, g.d_status + 1))) return r;
+    if ((r = stc_run(c, space, n, nb, c.d_codes, nullptr, c.d_k, c.d_b, c.d_status + 1))) return r;
     int st;
-    if ((r = stc_copy_back(ni, nb, aloc, ld_aloc, bloc, aschur, bschur, st))) return r;
-    return st < 0 ? st : g.h_status[0];
+    if ((r = stc_copy_back(c, ni, nb, aloc, ld_aloc, bloc, aschur, bschur, st))) return r;
+    return st < 0 ? st : c.h_status[0];
 }
 
 } // extern "C"

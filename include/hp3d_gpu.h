@@ -72,14 +72,21 @@ extern "C" {
 
 /* ------------------------------------------------------------ lifecycle */
 
-/* Select the device and create the library's stream and pinned staging for
- * the per-element entries, sized for elements of up to max_ndof dofs (0 = the
- * MAXP maximum, 882); later per-element calls allocate nothing and reject a
- * larger element with HP3D_E_ARG.  Replaces the THREADPRIVATE workspace setup of mumps_sc Step 2
- * (omp_mumps/mumps_sc.F90:228-251). */
+/* Select the device and fix the size of the per-element entries' staging:
+ * elements of up to max_ndof dofs (0 = the MAXP maximum, 882); a larger
+ * element is rejected with HP3D_E_ARG.  Every host thread that calls a
+ * per-element entry gets its own context (a stream, device staging and pinned
+ * host staging of about 5 max_ndof^2 doubles), made on its first call after
+ * init (the calling thread's now), with the library's device made current on
+ * that thread; later calls on the thread allocate nothing.  Calls from
+ * different threads therefore run concurrently on the GPU.  This replaces the
+ * THREADPRIVATE workspace setup of mumps_sc Step 2 (omp_mumps/mumps_sc.F90:
+ * 228-251).  hp3d_gpu_finalize frees every context; call it when no other
+ * call is in flight. */
 int hp3d_gpu_init(int device, int max_ndof);
 int hp3d_gpu_finalize(void);
-/* last HIP error string (for messages) */
+/* the calling thread's last error message (a HIP error string, or which
+ * argument or state was refused) */
 const char *hp3d_gpu_last_error(void);
 
 /* Process-wide launch options, read when a batched launch is made (a

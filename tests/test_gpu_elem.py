@@ -467,3 +467,40 @@ def test_concurrent_bins_match_sequential(gpu, precision):
     batch.check_status()
     for key in o1:
         assert torch.equal(o1[key], o2[key]), key
+
+
+def test_per_element_entries_from_threads(gpu):
+    """The per-element entries called from 8 host threads at once (an OpenMP
+    element loop): each thread runs on its own context, and every result is
+    bit-identical to the same call made alone.  The threaded loop's rate
+    against the one-thread loop is printed."""
+    import time
+    from concurrent.futures import ThreadPoolExecutor
+
+    import hp3d_amd as he
+
+    rng = np.random.default_rng(12)
+    po = rng.integers(1, 5, size=(48, 3))
+    m = mesh.structured((4, 4, 3), porder=po, perturb=0.1, mesh_seed=1, perm_seed=2,
+                        coef=np.exp(rng.normal(0, 0.5, 48)))
+    xn = m.xnod()
+
+    def one(e):
+        if e % 2:
+            return he.elem(0, m.norder[e], m.nori_e[e], m.nori_f[e], xn[e], coef=m.coef[e])
+        return he.celem(0, m.norder[e], m.nori_e[e], m.nori_f[e], xn[e], coef=m.coef[e])
+
+    t0 = time.perf_counter()
+    alone = [one(e) for e in range(m.nel)]
+    t1 = time.perf_counter()
+    with ThreadPoolExecutor(8) as ex:
+        first = list(ex.map(one, range(m.nel)))       # makes the 8 contexts
+        t2 = time.perf_counter()
+        second = list(ex.map(one, range(m.nel)))
+        t3 = time.perf_counter()
+    for e in range(m.nel):
+        for a, b, c in zip(alone[e], first[e], second[e]):
+            assert np.array_equal(np.asarray(a), np.asarray(b)), e
+            assert np.array_equal(np.asarray(a), np.asarray(c)), e
+    print("per-element loop: 1 thread %.0f el/s, 8 threads %.0f el/s (first pass, with the "
+          "contexts' set-up: %.0f el/s)" % (m.nel / (t1 - t0), m.nel / (t3 - t2), m.nel / (t2 - t1)))

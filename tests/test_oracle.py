@@ -407,3 +407,105 @@ def test_blas_element_loop_matches_plain_loop(space):
         for e in range(m.nel):
             nrm = np.linalg.norm(a[key][e])
             assert np.linalg.norm(a[key][e] - b[key][e]) <= 1e-12 * max(nrm, 1e-300), (key, e)
+
+
+# the hexahedron's vertices in hp3D order (master element [0,1]^3)
+_VX = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
+                [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], float)
+
+
+def _codes(p, rng=None):
+    no = [p] * 12 + [11 * p] * 6 + [111 * p]
+    if rng is None:
+        return no, np.zeros(12, np.int32), np.zeros(6, np.int32)
+    return no, rng.integers(0, 2, 12), rng.integers(0, 8, 6)
+
+
+def _fit(shape_fn, no, ne, nf, target, ncomp, npts=400, seed=0):
+    """Coefficients c with sum_k c_k phi_k = target on the master element,
+    least squares at random points (the shape functions are pinned by the
+    golden vectors; this recovers a known function's dofs)."""
+    xi = np.random.default_rng(seed).uniform(0, 1, (npts, 3))
+    rows, rhs = [], []
+    for x in xi:
+        s = np.asarray(shape_fn(x, no, ne, nf)[0]).reshape(-1, ncomp)
+        rows.append(s.T)
+        rhs.append(np.atleast_1d(target(x)))
+    a, b = np.concatenate(rows), np.concatenate(rhs)
+    c = np.linalg.lstsq(a, b, rcond=None)[0]
+    assert np.abs(a @ c - b).max() < 1e-12       # the function lies in the space
+    return c
+
+
+def test_elem_mass_closed_forms_on_a_box():
+    """orc_elem_mass on a box element of sides h: the vertex block is the Q1
+    tensor product h_x h_y h_z prod_d m(a_d, b_d) with m = 1/3 (same end),
+    1/6 (other end); c^T M c = int u^2 for u = xi_x^2 xi_y (p = 2, any
+    orientation); H(curl): c^T M c = int |E|^2 for the covariant field
+    E = J^-T (xi_y, 0, 0) (p = 1) and J^-T (xi_x xi_z^2, 0, 0) (p = 2)."""
+    h = np.array([0.7, 1.3, 0.4])
+    xnod = (np.array([0.2, -0.1, 0.5]) + _VX * h).T
+    vol = np.prod(h)
+    m1 = np.array([[1 / 3, 1 / 6], [1 / 6, 1 / 3]])
+    q1 = np.array([[np.prod([m1[int(_VX[a, d]), int(_VX[b, d])] for d in range(3)])
+                    for b in range(8)] for a in range(8)]) * vol
+    rng = np.random.default_rng(5)
+    for p in (1, 3):
+        no, ne, nf = _codes(p, rng)
+        m = oracle.elem_mass(0, no, ne, nf, xnod)
+        assert np.abs(m[:8, :8] - q1).max() < 1e-15
+    no, ne, nf = _codes(2, rng)
+    c = _fit(oracle.shape3dh, no, ne, nf, lambda x: x[0] ** 2 * x[1], 1)
+    m = oracle.elem_mass(0, no, ne, nf, xnod)
+    assert abs(c @ m @ c - vol / 15) < 1e-14
+    for p, target, integral in ((1, lambda x: [x[1], 0, 0], 1 / 3),
+                                (2, lambda x: [x[0] * x[2] ** 2, 0, 0], 1 / 15)):
+        no, ne, nf = _codes(p, rng)
+        c = _fit(oracle.shape3de, no, ne, nf, target, 3)
+        m = oracle.elem_mass(1, no, ne, nf, xnod)
+        assert abs(c @ m @ c - h[1] * h[2] / h[0] * integral) < 1e-14
+
+
+def test_elem_mass_volume_of_a_trilinear_element():
+    """1^T M 1 (u = 1: vertex coefficients 1) is the element's volume, here
+    int det J of the trilinear map by the 2-point Gauss rule per direction,
+    which integrates det J (degree <= 2 in each variable) exactly."""
+    rng = np.random.default_rng(8)
+    X = _VX + rng.uniform(-0.15, 0.15, (8, 3))
+    g = np.array([0.5 - 0.5 / np.sqrt(3), 0.5 + 0.5 / np.sqrt(3)])
+    vol = 0.0
+    for xi in np.array(np.meshgrid(g, g, g, indexing="ij")).reshape(3, -1).T:
+        f = np.where(_VX == 1, xi, 1 - xi)                # (8, 3) 1D factors
+        df = np.where(_VX == 1, 1.0, -1.0)
+        dn = np.stack([df[:, 0] * f[:, 1] * f[:, 2], f[:, 0] * df[:, 1] * f[:, 2],
+                       f[:, 0] * f[:, 1] * df[:, 2]], 1)  # dN_a / dxi_d
+        vol += np.linalg.det(X.T @ dn) / 8
+    for p in (1, 4):
+        no, ne, nf = _codes(p, rng)
+        m = oracle.elem_mass(0, no, ne, nf, X.T)
+        one = np.zeros(m.shape[0])
+        one[:8] = 1
+        assert abs(one @ m @ one - vol) < 1e-14 * vol
+
+
+@pytest.mark.parametrize("split,tol", [(1, 2e-8), (2, 1e-13), (3, 1e-13)])
+def test_element_error_closed_forms(split, tol):
+    """element_error with a zero solution returns the exact solution's norm
+    twice: on the unit cube (whole, or as split^3 elements) int |grad u|^2 +
+    u^2 = 3 pi^2 / 8 + 1 / 8 for u = sin(pi x) sin(pi y) sin(pi z), and
+    int |curl E|^2 + |E|^2 = pi^2 / 2 + 1 / 4 for E = (sin(pi y) sin(pi z), 0, 0).
+    The rule at p = 5 has 7 Gauss points per direction: a whole period of
+    sin^2 leaves a quadrature error of 9e-9; half a period or less, none
+    visible in FP64."""
+    for space, sel, expect, p in ((0, oracle.LOAD_SINSINSIN, 3 * np.pi ** 2 / 8 + 1 / 8, 5),
+                                  (1, oracle.LOAD_CURL_SIN, np.pi ** 2 / 2 + 1 / 4, 5)):
+        no, ne, nf = _codes(p)
+        n = oracle.ndof_h1(no) if space == 0 else oracle.ndof_hcurl(no)
+        err = nrm = 0.0
+        for corner in np.array(np.meshgrid(*[np.arange(split)] * 3, indexing="ij")).reshape(3, -1).T:
+            xnod = ((corner + _VX) / split).T
+            de, dn = oracle.element_error(space, no, ne, nf, xnod, np.zeros(n), sel)
+            err += de
+            nrm += dn
+        assert abs(err - expect) < tol * expect, (space, err, expect)
+        assert abs(nrm - expect) < tol * expect
