@@ -2316,14 +2316,20 @@ __global__ __launch_bounds__(WG) void csr_spmv_kernel(long long nrows, const lon
 // a finalize is recognised by its generation and replaced on its next call.
 struct Ctx {
     hipStream_t stream = nullptr;
-    int *d_codes = nullptr; // norder[19], nori_e[12], nori_f[6]; n, nb for stc_fwd
-    double *d_xnod = nullptr, *d_coef = nullptr, *d_k = nullptr, *d_b = nullptr;
-    double *d_aii = nullptr, *d_bi = nullptr, *d_as = nullptr, *d_bs = nullptr, *d_ws = nullptr;
+    // one device buffer and its pinned host mirror, laid out so that one copy
+    // carries a call's inputs and one copy its outputs:
+    //   [0, IO_XA)            geometry dofs and coef, right-aligned at IO_XA
+    //   [IO_XA, +256)         header: status[0], codes from int 16 on
+    //   [IO_XA + 256, ...)    outputs (K and load, or Aii, Bi, ASchur, BSchur)
+    char *d_io = nullptr, *h_io = nullptr;
+    double *d_k = nullptr;  // K and load of the two-pass path; stc_fwd's input
+    double *h_k = nullptr;  // pinned staging of stc_fwd's input
+    double *d_ws = nullptr;
     long long *d_off = nullptr; // zeros
-    int *d_status = nullptr;
-    double *h_k = nullptr, *h_b = nullptr, *h_as = nullptr, *h_bs = nullptr, *h_x = nullptr;
-    int *h_codes = nullptr, *h_status = nullptr;
 };
+constexpr size_t IO_XA = ((3 * (size_t)MAXN_H1 + 1) * sizeof(double) + 255) / 256 * 256;
+constexpr size_t IO_HDR = 256;
+constexpr int IO_CODES = 16; // int offset of the codes in the header
 
 struct Lib {
     std::mutex mu;          // init, finalize and context creation
@@ -2497,14 +2503,15 @@ int hp3d_gpu_ndof(int space, const int norder[19], int *nrdof, int *nrdofb)
     return 0;
 }
 
+static size_t io_bytes() { return IO_XA + IO_HDR + ((size_t)g.max_ndof * g.max_ndof + 2 * g.max_ndof) * sizeof(double); }
+
 static void ctx_free(Ctx *c)
 {
     if (c->stream) hipStreamSynchronize(c->stream);
-    void *dev[] = {c->d_codes, c->d_xnod, c->d_coef, c->d_k, c->d_b, c->d_aii, c->d_bi,
-                   c->d_as,    c->d_bs,   c->d_ws,   c->d_off, c->d_status};
+    void *dev[] = {c->d_io, c->d_k, c->d_ws, c->d_off};
     for (void *p : dev)
         if (p) hipFree(p);
-    void *host[] = {c->h_k, c->h_b, c->h_as, c->h_bs, c->h_x, c->h_codes, c->h_status};
+    void *host[] = {c->h_io, c->h_k};
     for (void *p : host)
         if (p) hipHostFree(p);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -2516,30 +2523,15 @@ static int ctx_alloc(Ctx *c)
 {
     HIPCHK(hipSetDevice(g.device));
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    const int mx = g.max_ndof;
-    const size_t nn = (size_t)mx * mx;
-    HIPCHK(hipMalloc(&c->d_codes, 64 * sizeof(int)));
-    HIPCHK(hipMalloc(&c->d_xnod, 3 * (size_t)MAXN_H1 * sizeof(double)));
-    HIPCHK(hipMalloc(&c->d_coef, sizeof(double)));
-    HIPCHK(hipMalloc(&c->d_k, nn * sizeof(double)));
-    HIPCHK(hipMalloc(&c->d_b, mx * sizeof(double)));
-    HIPCHK(hipMalloc(&c->d_aii, nn * sizeof(double)));
-    HIPCHK(hipMalloc(&c->d_bi, mx * sizeof(double)));
-    HIPCHK(hipMalloc(&c->d_as, nn * sizeof(double)));
-    HIPCHK(hipMalloc(&c->d_bs, mx * sizeof(double)));
+    const size_t kb = ((size_t)g.max_ndof * g.max_ndof + g.max_ndof) * sizeof(double);
+    HIPCHK(hipMalloc(&c->d_io, io_bytes()));
+    HIPCHK(hipMalloc(&c->d_k, kb));
     HIPCHK(hipMalloc(&c->d_ws, std::max(stc_slab(rnd16(g.nbm)), geo_slab(HP3D_HCURL)) *
                                    sizeof(double)));
     HIPCHK(hipMalloc(&c->d_off, sizeof(long long)));
-    HIPCHK(hipMalloc(&c->d_status, 2 * sizeof(int)));
-    HIPCHK(hipHostMalloc(&c->h_k, nn * sizeof(double)));
-    HIPCHK(hipHostMalloc(&c->h_b, mx * sizeof(double)));
-    HIPCHK(hipHostMalloc(&c->h_as, nn * sizeof(double)));
-    HIPCHK(hipHostMalloc(&c->h_bs, mx * sizeof(double)));
-    HIPCHK(hipHostMalloc(&c->h_x, (3 * (size_t)MAXN_H1 + 1) * sizeof(double)));
-    HIPCHK(hipHostMalloc(&c->h_codes, 64 * sizeof(int)));
-    HIPCHK(hipHostMalloc(&c->h_status, 2 * sizeof(int)));
+    HIPCHK(hipHostMalloc(&c->h_io, io_bytes()));
+    HIPCHK(hipHostMalloc(&c->h_k, kb));
     HIPCHK(hipMemsetAsync(c->d_off, 0, sizeof(long long), c->stream));
-    HIPCHK(hipMemsetAsync(c->d_status, 0, 2 * sizeof(int), c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -2915,8 +2907,15 @@ int hp3d_gpu_csr_spmv(long long nrows, const long long *indptr, const int *indic
 
 // ---------------------------------------------------- per-element drop-in
 
-static int elem_one(Ctx &c, int space, const int norder[19], const int nori_e[12], const int nori_f[6],
-                    const double *xnod, int ngeo, double coef, int load, int &n, int &nb)
+static int *hdr_d(Ctx &c) { return (int *)(c.d_io + IO_XA); }
+static int *hdr_h(Ctx &c) { return (int *)(c.h_io + IO_XA); }
+static double *out_d(Ctx &c) { return (double *)(c.d_io + IO_XA + IO_HDR); }
+static double *out_h(Ctx &c) { return (double *)(c.h_io + IO_XA + IO_HDR); }
+
+// stage one element's codes, geometry and coef (status cleared) with one copy
+static int stage_element(Ctx &c, int space, const int norder[19], const int nori_e[12],
+                         const int nori_f[6], const double *xnod, int ngeo, double coef, int &n,
+                         int &nb, const double *&d_xnod)
 {
     if (!norder || !nori_e || !nori_f || !xnod || ngeo < 1 || ngeo > MAXN_H1) return HP3D_E_ARG;
     ElemOrders o;
@@ -2925,29 +2924,65 @@ static int elem_one(Ctx &c, int space, const int norder[19], const int nori_e[12
     int nh, nbh;
     dof_counts(HP3D_H1, o, nh, nbh);
     if (n > g.max_ndof || ngeo > nh) return HP3D_E_ARG;
-    memcpy(c.h_codes, norder, 19 * sizeof(int));
-    memcpy(c.h_codes + 19, nori_e, 12 * sizeof(int));
-    memcpy(c.h_codes + 31, nori_f, 6 * sizeof(int));
-    memcpy(c.h_x, xnod, 3 * (size_t)ngeo * sizeof(double));
-    c.h_x[3 * ngeo] = coef;
-    HIPCHK(hipMemcpyAsync(c.d_codes, c.h_codes, 37 * sizeof(int), hipMemcpyHostToDevice, c.stream));
-    HIPCHK(hipMemcpyAsync(c.d_xnod, c.h_x, 3 * (size_t)ngeo * sizeof(double), hipMemcpyHostToDevice, c.stream));
-    HIPCHK(hipMemcpyAsync(c.d_coef, c.h_x + 3 * ngeo, sizeof(double), hipMemcpyHostToDevice, c.stream));
-    int r = hp3d_gpu_elem_batch(space, HP3D_FP64, 1, c.d_codes, c.d_codes + 19, c.d_codes + 31,
-                                c.d_xnod, ngeo, c.d_coef, load, c.d_off, 0, c.d_k, c.d_off, c.d_b,
-                                c.d_ws, c.d_status, n, c.stream);
-    return r;
+    int *h = hdr_h(c);
+    h[0] = 0;
+    memcpy(h + IO_CODES, norder, 19 * sizeof(int));
+    memcpy(h + IO_CODES + 19, nori_e, 12 * sizeof(int));
+    memcpy(h + IO_CODES + 31, nori_f, 6 * sizeof(int));
+    const size_t xb = (3 * (size_t)ngeo + 1) * sizeof(double);
+    double *hx = (double *)(c.h_io + IO_XA - xb);
+    memcpy(hx, xnod, 3 * (size_t)ngeo * sizeof(double));
+    hx[3 * ngeo] = coef;
+    HIPCHK(hipMemcpyAsync(c.d_io + IO_XA - xb, c.h_io + IO_XA - xb, xb + IO_HDR,
+                          hipMemcpyHostToDevice, c.stream));
+    d_xnod = (const double *)(c.d_io + IO_XA - xb);
+    return 0;
 }
 
-static int copy_back_matrix(Ctx &c, double *dst, int ld, const double *src_dev, int rows, int cols,
-                            int ld_src)
+// one copy back of the header and the first `nout` doubles of the outputs
+static int fetch_outputs(Ctx &c, size_t nout)
 {
-    HIPCHK(hipMemcpyAsync(c.h_k, src_dev, (size_t)ld_src * cols * sizeof(double),
+    HIPCHK(hipMemcpyAsync(c.h_io + IO_XA, c.d_io + IO_XA, IO_HDR + nout * sizeof(double),
                           hipMemcpyDeviceToHost, c.stream));
     HIPCHK(hipStreamSynchronize(c.stream));
-    for (int j = 0; j < cols; ++j)
-        memcpy(dst + (size_t)j * ld, c.h_k + (size_t)j * ld_src, rows * sizeof(double));
     return 0;
+}
+
+static void put_matrix(double *dst, int ld, const double *src, int rows, int cols)
+{
+    for (int j = 0; j < cols; ++j) memcpy(dst + (size_t)j * ld, src + (size_t)j * rows, rows * sizeof(double));
+}
+
+// K and load of the staged element into the outputs (K packed, then the load)
+static int elem_staged(Ctx &c, int space, int n, int ngeo, const double *d_xnod, int load)
+{
+    const int *codes = hdr_d(c) + IO_CODES;
+    return hp3d_gpu_elem_batch(space, HP3D_FP64, 1, codes, codes + 19, codes + 31, d_xnod, ngeo,
+                               d_xnod + 3 * ngeo, load, c.d_off, 0, out_d(c), c.d_off,
+                               out_d(c) + (size_t)n * n, c.d_ws, hdr_d(c), n, c.stream);
+}
+
+static int copy_elem_outputs(Ctx &c, int n, double *aloc, int ld_aloc, double *bloc)
+{
+    if (int r = fetch_outputs(c, (size_t)n * n + n)) return r;
+    put_matrix(aloc, ld_aloc, out_h(c), n, n);
+    memcpy(bloc, out_h(c) + (size_t)n * n, n * sizeof(double));
+    return hdr_h(c)[0];
+}
+
+// the condensed system, packed in the outputs as Aii (ni x ni), Bi, ASchur
+// (nb x ni), BSchur, into the caller's ALOC/BLOC and CLOC arrays
+static int copy_condensed(Ctx &c, int ni, int nb, double *aloc, int ld_aloc, double *bloc,
+                          double *aschur, double *bschur)
+{
+    const size_t ai = (size_t)ni * ni;
+    if (int r = fetch_outputs(c, ai + ni + (size_t)nb * ni + nb)) return r;
+    const double *o = out_h(c);
+    put_matrix(aloc, ld_aloc, o, ni, ni);
+    memcpy(bloc, o + ai, ni * sizeof(double));
+    if (aschur) memcpy(aschur, o + ai + ni, (size_t)nb * ni * sizeof(double));
+    if (bschur) memcpy(bschur, o + ai + ni + (size_t)nb * ni, nb * sizeof(double));
+    return hdr_h(c)[0];
 }
 
 int hp3d_gpu_elem(int space, const int norder[19], const int nori_e[12], const int nori_f[6],
@@ -2959,62 +2994,12 @@ int hp3d_gpu_elem(int space, const int norder[19], const int nori_e[12], const i
     if (int rc = ctx_get(&cp)) return rc;
     Ctx &c = *cp;
     int n, nb;
-    int r = elem_one(c, space, norder, nori_e, nori_f, xnod, ngeo, coef, load, n, nb);
+    const double *d_xnod;
+    int r = stage_element(c, space, norder, nori_e, nori_f, xnod, ngeo, coef, n, nb, d_xnod);
     if (r) return r;
     if (!aloc || !bloc || ld_aloc < n) return HP3D_E_ARG;
-    HIPCHK(hipMemcpyAsync(c.h_b, c.d_b, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-    HIPCHK(hipMemcpyAsync(c.h_status, c.d_status, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-    if ((r = copy_back_matrix(c, aloc, ld_aloc, c.d_k, n, n, n))) return r;
-    memcpy(bloc, c.h_b, n * sizeof(double));
-    return *c.h_status;
-}
-
-static int stc_run(Ctx &c, int space, int n, int nb, const int *d_norder, const int *d_nsz,
-                   const double *d_k, const double *d_b, int *d_status)
-{
-    StcArgs a;
-    a.space = space;
-    a.nel = 1;
-    a.ld_k = 0;
-    a.ld_c = 0;
-    a.nbp_max = rnd16(g.nbm);
-    a.n_max = g.max_ndof;
-    a.nb_max = g.nbm;
-    a.norder = d_norder;
-    a.nsz = d_nsz;
-    a.off_k = a.off_v = a.off_c = a.off_s = c.d_off;
-    a.kin = d_k;
-    a.bin = d_b;
-    a.aii = c.d_aii;
-    a.bi = c.d_bi;
-    a.aschur = c.d_as;
-    a.bschur = c.d_bs;
-    a.ws = c.d_ws;
-    a.status = d_status;
-    (void)n;
-    // the kernel keeps a negative-Jacobian warning it finds in its status
-    // slot: start from 0 so nothing of an earlier call survives
-    HIPCHK(hipMemsetAsync(d_status, 0, sizeof(int), c.stream));
-    launch_stc<double>(a, nb, c.stream);
-    HIPCHK(hipGetLastError());
-    return 0;
-}
-
-// copy the condensed system back into the caller's ALOC/BLOC and CLOC arrays
-static int stc_copy_back(Ctx &c, int ni, int nb, double *aloc, int ld_aloc, double *bloc, double *aschur,
-                         double *bschur, int &st)
-{
-    int r;
-    HIPCHK(hipMemcpyAsync(c.h_status + 1, c.d_status + 1, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-    HIPCHK(hipMemcpyAsync(c.h_b, c.d_bi, ni * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-    HIPCHK(hipMemcpyAsync(c.h_bs, c.d_bs, nb * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-    HIPCHK(hipMemcpyAsync(c.h_as, c.d_as, (size_t)nb * ni * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-    if ((r = copy_back_matrix(c, aloc, ld_aloc, c.d_aii, ni, ni, ni))) return r;
-    memcpy(bloc, c.h_b, ni * sizeof(double));
-    if (aschur) memcpy(aschur, c.h_as, (size_t)nb * ni * sizeof(double));
-    if (bschur) memcpy(bschur, c.h_bs, nb * sizeof(double));
-    st = c.h_status[1];
-    return 0;
+    if ((r = elem_staged(c, space, n, ngeo, d_xnod, load))) return r;
+    return copy_elem_outputs(c, n, aloc, ld_aloc, bloc);
 }
 
 int hp3d_gpu_stc_fwd(int ni, int nb, double *aloc, int ld_aloc, double *bloc, double *aschur,
@@ -3028,19 +3013,39 @@ int hp3d_gpu_stc_fwd(int ni, int nb, double *aloc, int ld_aloc, double *bloc, do
     if (nb == 0) return 0;
     if (ni == 0) return HP3D_E_NOIFACE;
     if (nb > max_bubbles(HP3D_HCURL, g.max_ndof)) return HP3D_E_ARG;
-    // stage ALOC (ld_aloc) and BLOC into packed device buffers
+    // ALOC (ld_aloc) packed and BLOC behind it: one copy; the header (status
+    // cleared, sizes n and nb): a second
     for (int j = 0; j < n; ++j) memcpy(c.h_k + (size_t)j * n, aloc + (size_t)j * ld_aloc, n * sizeof(double));
-    memcpy(c.h_b, bloc, n * sizeof(double));
-    c.h_codes[0] = n;
-    c.h_codes[1] = nb;
-    HIPCHK(hipMemcpyAsync(c.d_k, c.h_k, (size_t)n * n * sizeof(double), hipMemcpyHostToDevice, c.stream));
-    HIPCHK(hipMemcpyAsync(c.d_b, c.h_b, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
-    HIPCHK(hipMemcpyAsync(c.d_codes, c.h_codes, 2 * sizeof(int), hipMemcpyHostToDevice, c.stream));
-    int r = stc_run(c, HP3D_H1, n, nb, nullptr, c.d_codes, c.d_k, c.d_b, c.d_status + 1);
-    if (r) return r;
-    int st;
-    if ((r = stc_copy_back(c, ni, nb, aloc, ld_aloc, bloc, aschur, bschur, st))) return r;
-    return st;
+    memcpy(c.h_k + (size_t)n * n, bloc, n * sizeof(double));
+    int *h = hdr_h(c);
+    h[0] = 0;
+    h[IO_CODES] = n;
+    h[IO_CODES + 1] = nb;
+    HIPCHK(hipMemcpyAsync(c.d_k, c.h_k, ((size_t)n * n + n) * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    HIPCHK(hipMemcpyAsync(c.d_io + IO_XA, c.h_io + IO_XA, IO_HDR, hipMemcpyHostToDevice, c.stream));
+    StcArgs a;
+    a.space = HP3D_H1;
+    a.nel = 1;
+    a.ld_k = 0;
+    a.ld_c = 0;
+    a.nbp_max = rnd16(nb);
+    a.n_max = n;
+    a.nb_max = nb;
+    a.norder = nullptr;
+    a.nsz = hdr_d(c) + IO_CODES;
+    a.off_k = a.off_v = a.off_c = a.off_s = c.d_off;
+    a.kin = c.d_k;
+    a.bin = c.d_k + (size_t)n * n;
+    const size_t ai = (size_t)ni * ni;
+    a.aii = out_d(c);
+    a.bi = out_d(c) + ai;
+    a.aschur = out_d(c) + ai + ni;
+    a.bschur = out_d(c) + ai + ni + (size_t)nb * ni;
+    a.ws = c.d_ws;
+    a.status = hdr_d(c);
+    launch_stc<double>(a, nb, c.stream);
+    HIPCHK(hipGetLastError());
+    return copy_condensed(c, ni, nb, aloc, ld_aloc, bloc, aschur, bschur);
 }
 
 int hp3d_gpu_celem(int space, const int norder[19], const int nori_e[12], const int nori_f[6],
@@ -3052,22 +3057,27 @@ int hp3d_gpu_celem(int space, const int norder[19], const int nori_e[12], const 
     if (int rc = ctx_get(&cp)) return rc;
     Ctx &c = *cp;
     int n, nb;
-    int r = elem_one(c, space, norder, nori_e, nori_f, xnod, ngeo, coef, load, n, nb);
+    const double *d_xnod;
+    int r = stage_element(c, space, norder, nori_e, nori_f, xnod, ngeo, coef, n, nb, d_xnod);
     if (r) return r;
     if (!aloc || !bloc || ld_aloc < n) return HP3D_E_ARG;
-    int ni = n - nb;
-    HIPCHK(hipMemcpyAsync(c.h_status, c.d_status, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     if (nb == 0) {
-        HIPCHK(hipMemcpyAsync(c.h_b, c.d_b, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-        if ((r = copy_back_matrix(c, aloc, ld_aloc, c.d_k, n, n, n))) return r;
-        memcpy(bloc, c.h_b, n * sizeof(double));
-        return c.h_status[0];
+        if ((r = elem_staged(c, space, n, ngeo, d_xnod, load))) return r;
+        return copy_elem_outputs(c, n, aloc, ld_aloc, bloc);
     }
+    const int ni = n - nb;
     if (ni == 0) return HP3D_E_NOIFACE;
-    if ((r = stc_run(c, space, n, nb, c.d_codes, nullptr, c.d_k, c.d_b, c.d_status + 1))) return r;
-    int st;
-    if ((r = stc_copy_back(c, ni, nb, aloc, ld_aloc, bloc, aschur, bschur, st))) return r;
-    return st < 0 ? st : c.h_status[0];
+    // the batched entry on a batch of one: the fused kernel for N <= 128 and
+    // nb <= 32, else the element pass and the strip condensation
+    const int *codes = hdr_d(c) + IO_CODES;
+    const size_t ai = (size_t)ni * ni;
+    double *o = out_d(c);
+    r = hp3d_gpu_celem_batch(space, HP3D_FP64, 1, codes, codes + 19, codes + 31, d_xnod, ngeo,
+                             d_xnod + 3 * ngeo, load, c.d_off, 0, c.d_k, c.d_off,
+                             c.d_k + (size_t)n * n, c.d_off, o, o + ai, c.d_off, o + ai + ni,
+                             o + ai + ni + (size_t)nb * ni, c.d_ws, hdr_d(c), n, nb, c.stream);
+    if (r) return r;
+    return copy_condensed(c, ni, nb, aloc, ld_aloc, bloc, aschur, bschur);
 }
 
 } // extern "C"

@@ -169,23 +169,36 @@ def test_fused_matches_two_pass(gpu, precision):
         assert rel(batch.k_of(o1, e), batch.k_of(o2, e)) == 0.0
 
 
-def test_dropin_elem_and_stc(gpu):
+@pytest.mark.parametrize("space,p", [(0, 3), (0, 5), (1, 3)])
+def test_dropin_elem_and_stc(gpu, space, p):
+    """The per-element entries (host arrays): elem, stc_fwd and celem, whose
+    batch of one runs the fused kernel (H1 p = 3) or the element pass and the
+    strip condensation (H1 p = 5, H(curl) p = 3)."""
     import hp3d_amd
 
-    m = mesh.structured((2, 2, 2), porder=3, perturb=0.1, mesh_seed=1, perm_seed=3)
+    m = mesh.structured((2, 2, 2), porder=p, perturb=0.1, mesh_seed=1, perm_seed=3,
+                        coef=np.exp(np.random.default_rng(p).normal(0, 0.5, 8)))
+    load = oracle.LOAD_SINSINSIN if space == 0 else oracle.LOAD_CURL_SIN
     for e in range(m.nel):
         args = (m.norder[e], m.nori_e[e], m.nori_f[e])
-        k, b, flag = hp3d_amd.elem(0, *args, m.xnod()[e], coef=1.0)
-        kr, br, _ = oracle.elem_h1(*args, m.xnod()[e].T, kappa=1.0)
+        c = m.coef[e]
+        k, b, flag = hp3d_amd.elem(space, *args, m.xnod()[e], coef=c, load=load)
+        if space == 0:
+            kr, br, _ = oracle.elem_h1(*args, m.xnod()[e].T, kappa=c, load=load)
+            nb = oracle.nbubble_h1(m.norder[e][18])
+        else:
+            kr, br, _ = oracle.elem_hcurl(*args, m.xnod()[e].T, eps=c, load=load)
+            nb = oracle.nbubble_hcurl(m.norder[e][18])
         assert flag == 0
         assert rel(k, kr) < TOL64 and rel(b, br) < TOL64
-        nb = oracle.nbubble_h1(m.norder[e][18])
         aii, bi, asc, bsc = hp3d_amd.stc_fwd(kr, br, nb)
         ar, bir, ascr, bscr = oracle.stc_fwd(kr, br, nb, herm=False)
         assert rel(aii, ar) < TOL64 and rel(bi, bir) < TOL64
         assert rel(asc, ascr) < TOL64 and rel(bsc, bscr) < TOL64
-        aii2, bi2, asc2, bsc2, fl2 = hp3d_amd.celem(0, *args, m.xnod()[e], coef=1.0)
-        assert rel(aii2, ar) < TOL64 and rel(asc2, ascr) < TOL64
+        aii2, bi2, asc2, bsc2, fl2 = hp3d_amd.celem(space, *args, m.xnod()[e], coef=c, load=load)
+        assert fl2 == 0
+        assert rel(aii2, ar) < TOL64 and rel(bi2, bir) < TOL64
+        assert rel(asc2, ascr) < TOL64 and rel(bsc2, bscr) < TOL64
 
 
 def test_order_bins_mixed_mesh_and_graph_replay(gpu):
@@ -490,17 +503,21 @@ def test_per_element_entries_from_threads(gpu):
             return he.elem(0, m.norder[e], m.nori_e[e], m.nori_f[e], xn[e], coef=m.coef[e])
         return he.celem(0, m.norder[e], m.nori_e[e], m.nori_f[e], xn[e], coef=m.coef[e])
 
-    t0 = time.perf_counter()
     alone = [one(e) for e in range(m.nel)]
-    t1 = time.perf_counter()
     with ThreadPoolExecutor(8) as ex:
         first = list(ex.map(one, range(m.nel)))       # makes the 8 contexts
-        t2 = time.perf_counter()
         second = list(ex.map(one, range(m.nel)))
-        t3 = time.perf_counter()
+        # rates over 10 passes of the mesh
+        reps = list(range(m.nel)) * 10
+        t0 = time.perf_counter()
+        for e in reps:
+            one(e)
+        t1 = time.perf_counter()
+        list(ex.map(one, reps))
+        t2 = time.perf_counter()
     for e in range(m.nel):
         for a, b, c in zip(alone[e], first[e], second[e]):
             assert np.array_equal(np.asarray(a), np.asarray(b)), e
             assert np.array_equal(np.asarray(a), np.asarray(c)), e
-    print("per-element loop: 1 thread %.0f el/s, 8 threads %.0f el/s (first pass, with the "
-          "contexts' set-up: %.0f el/s)" % (m.nel / (t1 - t0), m.nel / (t3 - t2), m.nel / (t2 - t1)))
+    print("per-element loop: 1 thread %.0f el/s, 8 threads %.0f el/s"
+          % (len(reps) / (t1 - t0), len(reps) / (t2 - t1)))
