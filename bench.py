@@ -285,7 +285,7 @@ def kernel_names(batch):
         else:
             ek = ("elem_single_kernel<float>" if mnd <= 128 else
                   "geometry_kernel + elem_kernel<float>")
-            parts.append("%s + stc_blocked_kernel<float> (%d elements)" % (ek, cnt))
+            parts.append("%s + stc_strip_kernel<float> (%d elements)" % (ek, cnt))
     return " ; ".join(parts)
 
 

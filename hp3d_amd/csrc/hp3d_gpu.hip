@@ -870,17 +870,19 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
     constexpr int NC = SPACE == 0 ? 3 : 6;
     constexpr int KCS = SingleCfg<T>::KCS;
     constexpr int QC = KCS / NC;               // points per chunk
-    constexpr int GQ = 64;                     // points per geometry pass (multiple of QC)
+    constexpr int GQ = SPACE == 0 ? 128 : 64;  // points per geometry pass (multiple of QC)
     constexpr int GS = SPACE == 0 ? 10 : 21;
     static_assert(R == BR || R == BR / 2, "128 or 64 rows");
-    constexpr int LD1 = R + 16;                // LDS stride of an A column and of a bubble row
+    constexpr int LD1 = R + 16;                // LDS stride of an A column (in T)
+    constexpr int LDK = LD1 + 1;               // LDS stride of a bubble row (doubles; odd, so
+                                               // that a column read across rows is conflict-free)
     constexpr int NBX = R == BR ? NBF : NBF / 2; // largest bubble block condensed here
     constexpr int LW = NBX + 1;                // leading dimension of L_u^-1
     constexpr int NSLOT = R == BR ? 9 : 3;     // tile slots per wave
     constexpr int UMAX = R == BR ? 5 : 2;      // right-hand-side tiles per wave in the condensation
     constexpr int BUF = KCS * LD1;             // one A buffer, in T
     constexpr int AC_D = (2 * BUF * (int)sizeof(T) + 7) / 8;
-    constexpr int CW_D = NBX * LD1 + NBX * LW + NBX + LD1 + 4 * 16 * 17; // output / condensation workspace
+    constexpr int CW_D = NBX * LDK + NBX * LW + NBX + LD1 + 4 * 16 * 17; // output / condensation workspace
     constexpr int SM_D = AC_D > CW_D ? AC_D : CW_D;
     static_assert(GQ % QC == 0 && AC_D >= QC * R, "geometry passes hold whole chunks");
 
@@ -1106,7 +1108,7 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
     // the threads share data through LDS only, and the barriers do not wait
     // for the K, Aii and ASchur stores)
     double *kb = smem;              // bubble rows b < nb: Abi cols 0..ni-1, Abb cols ni..n-1, Bb col n
-    double *wm = kb + NBX * LD1;    // L_u^-1, leading dimension LW
+    double *wm = kb + NBX * LDK;    // L_u^-1, leading dimension LW
     double *rs = wm + NBX * LW;     // D(j)^-1/2
     double *bv = rs + NBX;          // load vector
     double *stg = bv + LD1;         // 4 x 16 x 17 wave stages
@@ -1128,7 +1130,7 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
     if (sub == 0 && row_ok) {
         if (a.bout) a.bout[a.off_v[e] + row] = bsum;
         bv[row] = bsum;
-        if (cond && row >= ni) kb[(row - ni) * LD1 + n] = bsum;
+        if (cond && row >= ni) kb[(row - ni) * LDK + n] = bsum;
     }
     auto write_k = [&]() {
         T *kout = (T *)a.kout + a.off_k[e];
@@ -1159,8 +1161,8 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
             int r = ti * 16 + M::row(lane, j), c = tj * 16 + (lane & 15);
             double v = CY::val(hi[s], acc[s], j);
             if (r < n && c < n) {
-                if (c >= ni) kb[(c - ni) * LD1 + r] = v;
-                if (r >= ni) kb[(r - ni) * LD1 + c] = v;
+                if (c >= ni) kb[(c - ni) * LDK + r] = v;
+                if (r >= ni) kb[(r - ni) * LDK + c] = v;
             }
         }
     }
@@ -1170,7 +1172,7 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
     // own afterwards.  EW is a wave with the fewest tiles.
     constexpr int EW = 3;
     double *li = wm; // L_u^-1, leading dimension LW (unit lower; zero above)
-    if (wv == EW && eliminate_wave<NBX>(kb, LD1, ni, nb, li, LW, rs, lane) && lane == 0) s_fail = 1;
+    if (wv == EW && eliminate_wave<NBX>(kb, LDK, ni, nb, li, LW, rs, lane) && lane == 0) s_fail = 1;
     if (a.kout) write_k();
     lds_barrier();
     // GEMMs against the right-hand block [Abi | Bb] on FP64 MFMA.  Output
@@ -1186,11 +1188,20 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
             if (t >= ntile) continue;
             int tr = t / nct, tc = t % nct;
             int ar = tr * 16 + (lane & 15), pc = rhs_col(tc * 16 + (lane & 15));
-            for (int k0 = 0; k0 < nb; k0 += 4) {
-                int k = k0 + (lane >> 4);
-                double av = 0.0, bw = 0.0;
-                if (ar < nb && k < nb) av = second ? li[k * LW + ar] : li[ar * LW + k];
-                if (k < nb && pc >= 0) bw = second ? kb[k * LD1 + pc] * rs[k] : kb[k * LD1 + pc];
+            // loads from clamped indexes, masked at use: no predicated
+            // loads, and the fixed trip count lets them run ahead
+            const bool aok = ar < nb, bok = pc >= 0;
+            const int arc = aok ? ar : 0, pcc = bok ? pc : 0;
+#pragma unroll
+            for (int k0 = 0; k0 < NBX; k0 += 4) {
+                const int k = k0 + (lane >> 4);
+                const bool kok = k < nb;
+                const int kc = kok ? k : 0;
+                double av = second ? li[kc * LW + arc] : li[arc * LW + kc];
+                double bw = kb[kc * LDK + pcc];
+                if (second) bw *= rs[kc];
+                av = aok && kok ? av : 0.0;
+                bw = bok && kok ? bw : 0.0;
                 out[u] = MD::mma(av, bw, out[u]);
             }
         }
@@ -1205,7 +1216,7 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 int b = tr * 16 + MD::row(lane, j);
-                if (b < nb && pc >= 0) kb[b * LD1 + pc] = first ? in[u][j] * rs[b] : in[u][j];
+                if (b < nb && pc >= 0) kb[b * LDK + pc] = first ? in[u][j] * rs[b] : in[u][j];
             }
         }
     };
@@ -1219,7 +1230,7 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
     double *bi = cd.bi + a.off_v[e];
     if (tid < ni) {
         double sacc = bv[tid];
-        for (int k = 0; k < nb; ++k) sacc -= kb[k * LD1 + tid] * kb[k * LD1 + n];
+        for (int k = 0; k < nb; ++k) sacc -= kb[k * LDK + tid] * kb[k * LDK + n];
         bi[tid] = sacc;
     }
     // Aii = Kii - Z^T D^-1 Z = Kii - S^T S on the register tiles, in FP64
@@ -1234,11 +1245,17 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
 #pragma unroll
         for (int j = 0; j < 4; ++j) da[j] = CY::val(hi[s], acc[s], j);
         d4_t up = d4_t{0, 0, 0, 0};
-        for (int k0 = 0; k0 < nb; k0 += 4) {
-            int k = k0 + (lane >> 4);
-            int ra = R0 + (lane & 15), cb = C0 + (lane & 15);
-            double av = (k < nb && ra < ni) ? kb[k * LD1 + ra] : 0.0;
-            double bw = (k < nb && cb < ni) ? kb[k * LD1 + cb] : 0.0;
+        const int ra = R0 + (lane & 15), cb = C0 + (lane & 15);
+        const bool rok = ra < ni, cok = cb < ni;
+        const int rac = rok ? ra : 0, cbc = cok ? cb : 0;
+#pragma unroll
+        for (int k0 = 0; k0 < NBX; k0 += 4) {
+            const int k = k0 + (lane >> 4);
+            const bool kok = k < nb;
+            const int kc = kok ? k : 0;
+            double av = kb[kc * LDK + rac], bw = kb[kc * LDK + cbc];
+            av = kok && rok ? av : 0.0;
+            bw = kok && cok ? bw : 0.0;
             up = MD::mma(av, bw, up);
         }
         // up[j] is (S^T S)(R0 + MD::row(lane,j), C0 + lane%16): bring it to
@@ -1259,8 +1276,8 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
     lds_barrier();
     // ASchur (nb x ni, leading dimension nb) and BSchur, coalesced
     double *xs = cd.aschur + cd.off_s[e];
-    for (int idx = tid; idx < nb * ni; idx += WG) xs[idx] = kb[(idx % nb) * LD1 + idx / nb];
-    if (tid < nb) cd.bschur[a.off_v[e] + tid] = kb[tid * LD1 + n];
+    for (int idx = tid; idx < nb * ni; idx += WG) xs[idx] = kb[(idx % nb) * LDK + idx / nb];
+    if (tid < nb) cd.bschur[a.off_v[e] + tid] = kb[tid * LDK + n];
     if (tid == 0 && a.status) a.status[e] = s_fail ? HP3D_E_FACTOR : (es.negjac ? HP3D_NEG_JACOBIAN : 0);
 }
 
