@@ -870,7 +870,9 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
     constexpr int NC = SPACE == 0 ? 3 : 6;
     constexpr int KCS = SingleCfg<T>::KCS;
     constexpr int QC = KCS / NC;               // points per chunk
-    constexpr int GQ = SPACE == 0 ? 128 : 64;  // points per geometry pass (multiple of QC)
+    // points per geometry pass (a multiple of QC): 128 for the 128-row H1
+    // form, 64 where a third workgroup per CU has to fit in LDS
+    constexpr int GQ = SPACE == 0 && R == BR ? 128 : 64;
     constexpr int GS = SPACE == 0 ? 10 : 21;
     static_assert(R == BR || R == BR / 2, "128 or 64 rows");
     constexpr int LD1 = R + 16;                // LDS stride of an A column (in T)
