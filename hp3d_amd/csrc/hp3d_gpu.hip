@@ -1168,14 +1168,49 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
             }
         }
     }
-    lds_barrier();
     // Gaussian elimination of [Abb | I] in the registers of wave EW (L_u^-1
-    // and D^-1/2), while the other waves store their tiles of K; EW stores its
-    // own afterwards.  EW is a wave with the fewest tiles.
+    // and D^-1/2), while the other waves store their tiles of K.  In the
+    // FP32 128-row form EW first hands its own K tiles (the carry folded to
+    // FP32) to the other waves through the geometry buffer, free after the
+    // main loop: its stores would otherwise follow its elimination on the
+    // critical path.  Elsewhere EW stores its own afterwards.
     constexpr int EW = 3;
+    constexpr bool KHAND = sizeof(T) == 4 && R == BR;
+    T *khand = reinterpret_cast<T *>(geo);
+    static_assert(!KHAND || GQ * GS * 2 >= NSLOT * 256, "K hand-over buffer");
+    if constexpr (KHAND) {
+        if (a.kout && wv == EW) {
+#pragma unroll
+            for (int s = 0; s < NSLOT; ++s) {
+                if (!(valid & (1u << s))) continue;
+                const acc_t v = CY::sum(hi[s], acc[s]);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) khand[s * 256 + 4 * lane + j] = v[j];
+            }
+        }
+    }
+    lds_barrier();
     double *li = wm; // L_u^-1, leading dimension LW (unit lower; zero above)
     if (wv == EW && eliminate_wave<NBX>(kb, LDK, ni, nb, li, LW, rs, lane) && lane == 0) s_fail = 1;
-    if (a.kout) write_k();
+    if (a.kout) {
+        if (!(KHAND && wv == EW)) write_k();
+        if constexpr (KHAND) {
+            if (wv != EW) {
+                // EW's slots s = wv, wv + 3, wv + 6: tile row 3 (columns 3..7),
+                // then tile row 4 (columns 4..7)
+                T *kout = (T *)a.kout + a.off_k[e];
+                const size_t ld = a.ld_k > 0 ? (size_t)a.ld_k : (size_t)n;
+                for (int s = wv; s < NSLOT; s += 3) {
+                    const int ti = s < 5 ? 3 : 4, tj = s < 5 ? 3 + s : s - 1;
+                    if (ti * 16 >= n || tj * 16 >= n) continue;
+                    acc_t v;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) v[j] = khand[s * 256 + 4 * lane + j];
+                    write_sym_tile<acc_t, T, M>(v, ti * 16, tj * 16, n, kout, ld, kstage, lane);
+                }
+            }
+        }
+    }
     lds_barrier();
     // GEMMs against the right-hand block [Abi | Bb] on FP64 MFMA.  Output
     // tiles: rows 16*tr.. (bubble index), columns 16*tc.. (ni Abi columns,
