@@ -998,26 +998,27 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
 #pragma unroll
     for (int i = 0; i < RPT; ++i) bacc[i] = 0.0;
 
+    // Branch-free: beyond the chunk (lq >= gend) or N the entries come from
+    // point 0 and descriptor 0 (valid table indexes) and are stored as zeros,
+    // so no load sits under a predicate
     auto build = [&](int g0, int l0, int gend, T *buf) {
         const int lq = l0 + qt;
         const bool pok = lq < gend;
-        Pt p{0, 0, 0};
+        const int lqc = pok ? lq : 0;
+        const unsigned pk = pix[g0 + lqc];
+        const Pt p{(int)(pk & 15), (int)((pk >> 4) & 15), (int)(pk >> 8)};
         double g[GS];
-        if (pok) {
-            const unsigned pk = pix[g0 + lq];
-            p = Pt{(int)(pk & 15), (int)((pk >> 4) & 15), (int)(pk >> 8)};
 #pragma unroll
-            for (int k = 0; k < GS; ++k) g[k] = geo[lq * GS + k];
-        }
+        for (int k = 0; k < GS; ++k) g[k] = geo[lqc * GS + k];
 #pragma unroll 2
         for (int i = 0; i < RPT; ++i) {
             const int r = rt + TPP * i;
             double out[NC];
+            const double bl = a_entries_load<SPACE, true>(dsp[r], p, tv, td, g, out);
+            const bool ok = pok && r < n;
+            bacc[i] += ok ? bl : 0.0;
 #pragma unroll
-            for (int c = 0; c < NC; ++c) out[c] = 0.0;
-            if (pok && r < n) bacc[i] += a_entries_load<SPACE, true>(dsp[r], p, tv, td, g, out);
-#pragma unroll
-            for (int c = 0; c < NC; ++c) buf[(NC * qt + c) * LD1 + r] = (T)out[c];
+            for (int c = 0; c < NC; ++c) buf[(NC * qt + c) * LD1 + r] = ok ? (T)out[c] : (T)0;
         }
     };
     // The slot mask is wave-uniform.  When some of this wave's tiles lie beyond
