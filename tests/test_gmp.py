@@ -117,3 +117,26 @@ def test_solution_on_reference_geometry():
     m.verts = m.verts / 3.0
     r = sr.solve(m, 0, oracle.LOAD_POLY2)
     assert np.sqrt(r["error2"]) < 1e-14
+
+
+@pytest.mark.parametrize("p,tol", [(1, 1e-15), (3, 1e-14)])
+def test_poly_pois_on_reference_geometry(p, tol):
+    """test/poly_pois.F90's check (u = xyz with its Dirichlet data, f = 0, H1
+    error <= 1e-15) on the reference's own 27-hexa geometry scaled to the
+    unit cube, through the GMP orientation codes, the first-visit numbering
+    (equal to the test's own NFIRST_DOF restatement) and the Dirichlet
+    interpolation of every boundary node.  p = 1 is the test's own order and
+    bound; p = 3 (measured 2.2e-15) is held to 1e-14."""
+    from hp3d_amd import dirichlet
+    from hp3d_amd.solve import DofMap
+
+    m = gmp.read_gmp(GEOM, porder=p)
+    m.verts = m.verts / 3.0
+    dm = DofMap(m, 0, numbering="first_visit")
+    ref = sr.Numbering(m, 0)
+    assert dm.ndof == ref.ndof
+    for e in range(m.nel):
+        assert np.array_equal(dm.gid[e, :dm.ni[e]], ref.gid[e])
+    ud = dirichlet.dirichlet_values(m, dm, oracle.LOAD_XYZ)
+    r = sr.solve(m, 0, oracle.LOAD_ZERO, exact=oracle.LOAD_XYZ, ud=ud)
+    assert np.sqrt(r["error2"]) <= tol, np.sqrt(r["error2"])
