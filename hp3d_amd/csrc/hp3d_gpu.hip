@@ -744,16 +744,17 @@ __global__ __launch_bounds__(WGE, 4) void elem_kernel(ElemArgs a)
         for (int qq = 0; qq < NP; ++qq) {
             const int q = P * qq + sub, lq = l0 + q;
             if (QC % P != 0 && q >= QC) break;
+            // branch-free: past the chunk or N, point 0 and the row's (valid)
+            // descriptor give finite entries that are stored as zeros
+            const bool ok = row_ok && lq < gend;
+            const int lqc = lq < gend ? lq : 0;
+            const unsigned pk = pix[g0 + lqc];
+            const Pt p{(int)(pk & 15), (int)((pk >> 4) & 15), (int)(pk >> 8)};
             double out[NC];
+            const double bl = a_entries_load<SPACE>(drow, p, tv, td, geo + lqc * GS, out);
+            bsum += ok ? bl : 0.0;
 #pragma unroll
-            for (int c = 0; c < NC; ++c) out[c] = 0.0;
-            if (row_ok && lq < gend) {
-                unsigned pk = pix[g0 + lq];
-                Pt p{(int)(pk & 15), (int)((pk >> 4) & 15), (int)(pk >> 8)};
-                bsum += a_entries_load<SPACE>(drow, p, tv, td, geo + lq * GS, out);
-            }
-#pragma unroll
-            for (int c = 0; c < NC; ++c) buf[(NC * q + c) * LDA + row] = (T)out[c];
+            for (int c = 0; c < NC; ++c) buf[(NC * q + c) * LDA + row] = ok ? (T)out[c] : (T)0;
         }
     };
     auto build = [&](int g0, int l0, int gend, T *buf) {
