@@ -432,6 +432,17 @@ __device__ __forceinline__ double readlane_f64(double v, int l)
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
+// 1 / d for d > 0 from the hardware reciprocal estimate (v_rcp_f64) and two
+// Newton-Raphson steps r <- r + r (1 - d r): a few dependent operations where
+// the IEEE division is a dozen, on the elimination's critical path (the
+// result is within an ulp or two of 1 / d)
+__device__ __forceinline__ double recip_pos(double d)
+{
+    double r = __builtin_amdgcn_rcp(d);
+    r = fma(r, fma(-d, r, 1.0), r);
+    return fma(r, fma(-d, r, 1.0), r);
+}
+
 // The steps of eliminate_wave over the first NR rows of the lane's column
 // (nb <= NR; rows nb.. are padding, whose multiplier 0 leaves them as they
 // are).  Step j updates row j + 1 first and starts the reciprocal of the
@@ -442,7 +453,7 @@ __device__ __forceinline__ bool eliminate_rows(double (&a)[NBX], int nb, int lan
 {
     bool bad = false;
     const double piv0 = readlane_f64(a[0], 0);
-    double rp = piv0 > 0.0 ? 1.0 / piv0 : 0.0;
+    double rp = piv0 > 0.0 ? recip_pos(piv0) : 0.0;
     bad |= nb > 0 && !(piv0 > 0.0);
 #pragma unroll
     for (int j = 0; j < NR; ++j) {
@@ -453,7 +464,7 @@ __device__ __forceinline__ bool eliminate_rows(double (&a)[NBX], int nb, int lan
             if (j + 1 < NR) {
                 a[j + 1] = fma(-readlane_f64(a[j + 1], j), u, a[j + 1]);
                 const double pn = readlane_f64(a[j + 1], j + 1);
-                rpn = pn > 0.0 ? 1.0 / pn : 0.0;
+                rpn = pn > 0.0 ? recip_pos(pn) : 0.0;
                 bad |= j + 1 < nb && !(pn > 0.0);
             }
 #pragma unroll
