@@ -521,3 +521,26 @@ def test_per_element_entries_from_threads(gpu):
             assert np.array_equal(np.asarray(a), np.asarray(c)), e
     print("per-element loop: 1 thread %.0f el/s, 8 threads %.0f el/s"
           % (len(reps) / (t1 - t0), len(reps) / (t2 - t1)))
+
+
+def test_finalize_and_reinit_with_thread_contexts(gpu):
+    """hp3d_gpu_finalize frees every thread's context; after hp3d_gpu_init
+    again, threads that held one get a fresh context on their next call and
+    compute the same bits."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import hp3d_amd as he
+    from hp3d_amd import _lib
+
+    m = mesh.structured((2, 2, 2), porder=[[2, 3, 2]] * 8, perturb=0.1, mesh_seed=3, perm_seed=1)
+    xn = m.xnod()
+    one = lambda e: he.celem(0, m.norder[e], m.nori_e[e], m.nori_f[e], xn[e], coef=1.0 + e)
+    lib = _lib.load()
+    with ThreadPoolExecutor(4) as ex:
+        before = list(ex.map(one, range(m.nel)))
+        assert lib.hp3d_gpu_finalize() == 0
+        assert lib.hp3d_gpu_init(0, 0) == 0
+        after = list(ex.map(one, range(m.nel)))
+    for b, a in zip(before, after):
+        for x, y in zip(b, a):
+            assert np.array_equal(np.asarray(x), np.asarray(y))
