@@ -1319,15 +1319,29 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
         wave_sync_lds();
         write_sym_tile<d4_t, double, M>(da, R0, C0, ni, aii, (size_t)ni, st, lane);
     }
-    // X = L_u^-T D^-1/2 S = Abb^-1 [Abi | Bb]
-    if (nb > 0) gemm_li(true, yt);
-    lds_barrier();
-    if (nb > 0) store_rhs(false, yt);
-    lds_barrier();
-    // ASchur (nb x ni, leading dimension nb) and BSchur, coalesced
-    double *xs = cd.aschur + cd.off_s[e];
-    for (int idx = tid; idx < nb * ni; idx += WG) xs[idx] = kb[(idx % nb) * LDK + idx / nb];
-    if (tid < nb) cd.bschur[a.off_v[e] + tid] = kb[tid * LDK + n];
+    // X = L_u^-T D^-1/2 S = Abb^-1 [Abi | Bb], stored from the MFMA tiles
+    // straight into ASchur (nb x ni, leading dimension nb) and BSchur: the
+    // Aii step above and this one only read S, so neither waits for the other
+    if (nb > 0) {
+        gemm_li(true, yt);
+        double *xs = cd.aschur + cd.off_s[e];
+        double *xb = cd.bschur + a.off_v[e];
+#pragma unroll
+        for (int u = 0; u < UMAX; ++u) {
+            const int t = wave + 4 * u;
+            if (t >= ntile) continue;
+            const int tr = t / nct, tc = t % nct;
+            const int pc = rhs_col(tc * 16 + (lane & 15));
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int b = tr * 16 + MD::row(lane, j);
+                if (b < nb && pc >= 0) {
+                    if (pc < ni) xs[(size_t)pc * nb + b] = yt[u][j];
+                    else xb[b] = yt[u][j];
+                }
+            }
+        }
+    }
     if (tid == 0 && a.status) a.status[e] = s_fail ? HP3D_E_FACTOR : (es.negjac ? HP3D_NEG_JACOBIAN : 0);
 }
 
