@@ -1125,13 +1125,21 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
     double *kb = smem;              // bubble rows b < nb: Abi cols 0..ni-1, Abb cols ni..n-1, Bb col n
     double *wm = kb + NBX * LDK;    // L_u^-1, leading dimension LW
     double *rs = wm + NBX * LW;     // D(j)^-1/2
-    double *bv = rs + NBX;          // load vector
-    double *stg = bv + LD1;         // 4 x 16 x 17 wave stages
+    double *stg = rs + NBX + LD1;   // 4 x 16 x 17 wave stages
     T *kstage = reinterpret_cast<T *>(stg + wave * (16 * 17));
-    // the load vector: the QC partial sums of each row (in the free A buffers)
+    // the FP32 128-row form hands K tiles over through geo[0, NSLOT * 128)
+    // (as doubles' room); the load vector sits in geo after them
+    constexpr bool KHAND = sizeof(T) == 4 && R == BR;
+    constexpr int BV_OFF = KHAND ? NSLOT * 128 : 0;
+    static_assert(GQ * GS >= BV_OFF + R, "load vector beside the K hand-over");
+    double *bv = geo + BV_OFF;      // load vector
+    // the load vector: the QC partial sums of each row, in the L_u^-1, D and
+    // stage space (not read or written before the next barrier but one), so
+    // that the bubble rows can go into kb right after the sums are read
     double bsum = 0.0;
     {
-        double *bq = smem; // QC x R
+        static_assert(QC * R <= NBX * LW + NBX + LD1 + 4 * 16 * 17, "partial sums in the stage space");
+        double *bq = wm; // QC x R
 #pragma unroll
         for (int i = 0; i < RPT; ++i) bq[qt * R + rt + TPP * i] = bacc[i];
         lds_barrier();
@@ -1141,7 +1149,6 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
     }
     const int nb = es.nb, ni = n - nb;
     const bool cond = cd.aii != nullptr && nb <= NBX;
-    lds_barrier();
     if (sub == 0 && row_ok) {
         if (a.bout) a.bout[a.off_v[e] + row] = bsum;
         bv[row] = bsum;
@@ -1159,6 +1166,7 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
         }
     };
     if (!cond) {
+        lds_barrier(); // the partial sums are read before the stages are written
         if (a.kout) write_k();
         // an element beyond the bubble limit the caller declared is an error
         if (tid == 0 && a.status)
@@ -1188,7 +1196,6 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
     // main loop: its stores would otherwise follow its elimination on the
     // critical path.  Elsewhere EW stores its own afterwards.
     constexpr int EW = 3;
-    constexpr bool KHAND = sizeof(T) == 4 && R == BR;
     T *khand = reinterpret_cast<T *>(geo);
     static_assert(!KHAND || GQ * GS * 2 >= NSLOT * 256, "K hand-over buffer");
     if constexpr (KHAND) {
