@@ -537,6 +537,28 @@ __device__ inline void write_sym_tile(V acc, int R, int C, int lim, S *out, size
 
 
 
+// write_sym_tile without the LDS stage: the mirrored tile as there
+// (coalesced over the 16 lanes of a row group), the tile itself as this
+// lane's four entries of column C + lane % 16 (consecutive rows in the FP32
+// layout: one 16- or 32-byte run per lane)
+template <typename V, typename S, typename M>
+__device__ inline void write_sym_tile_direct(V acc, int R, int C, int lim, S *out, size_t ld, int lane)
+{
+    const int rr = C + (lane & 15);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int cc = R + M::row(lane, j);
+        if (rr < lim && cc < lim) out[(size_t)cc * ld + rr] = (S)acc[j];
+    }
+    if (R != C) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int r2 = R + M::row(lane, j);
+            if (r2 < lim && rr < lim) out[(size_t)rr * ld + r2] = (S)acc[j];
+        }
+    }
+}
+
 // FP32 accumulation with an error-free carry: every ~96 A columns the MFMA
 // accumulator c is folded into a running FP32 sum h by TwoSum
 // (h' = fl(h + c), c' = (h + c) - h' exactly), so c only ever accumulates a
@@ -1324,7 +1346,7 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
 #pragma unroll
         for (int j = 0; j < 4; ++j) da[j] -= st[M::row(lane, j) * 17 + (lane & 15)];
         wave_sync_lds();
-        write_sym_tile<d4_t, double, M>(da, R0, C0, ni, aii, (size_t)ni, st, lane);
+        write_sym_tile_direct<d4_t, double, M>(da, R0, C0, ni, aii, (size_t)ni, lane);
     }
     // X = L_u^-T D^-1/2 S = Abb^-1 [Abi | Bb], stored from the MFMA tiles
     // straight into ASchur (nb x ni, leading dimension nb) and BSchur: the
