@@ -1044,7 +1044,7 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
         double g[GS];
 #pragma unroll
         for (int k = 0; k < GS; ++k) g[k] = geo[lqc * GS + k];
-#pragma unroll 2
+#pragma unroll 4
         for (int i = 0; i < RPT; ++i) {
             const int r = rt + TPP * i;
             double out[NC];
