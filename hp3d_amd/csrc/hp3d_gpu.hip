@@ -370,11 +370,10 @@ __device__ __forceinline__ double load_entry(unsigned d, Pt p, const double (*tv
 // element header: orders, counts, quadrature sizes (thread 0), plus status on
 // failure.  Returns false (uniformly) if the element is invalid.
 template <int SPACE>
-__device__ __forceinline__ bool element_setup(const ElemArgs &a, int e, ElemShared &es)
+__device__ __forceinline__ bool element_setup(const ElemArgs &a, int e, ElemShared &es, const int t = threadIdx.x)
 {
     // the 37 header integers (norder, nori_e, nori_f) in one parallel load
     __shared__ int hdr[40];
-    const int t = threadIdx.x;
     if (t < 19) hdr[t] = a.norder[19 * (size_t)e + t];
     else if (t < 31) hdr[t] = a.nori_e[12 * (size_t)e + (t - 19)];
     else if (t < 37) hdr[t] = a.nori_f[6 * (size_t)e + (t - 31)];
@@ -896,7 +895,8 @@ template <typename T> struct SingleCfg {
 };
 
 template <typename T, int SPACE, int R>
-__device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs &cd, const int e)
+__device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs &cd, const int e,
+                                               const int tid)
 {
     typedef Mfma<T> M;
     typedef typename M::acc_t acc_t;
@@ -933,8 +933,16 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
     __shared__ int s_fail;
     T *ac = reinterpret_cast<T *>(smem);
 
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    if (!element_setup<SPACE>(a, e, es)) {
+    const int lane = tid & 63, wave = tid >> 6;
+    // the element's geometry nodes (up to 27-node maps, one value per thread)
+    // and its coefficient are loaded before the header is decoded, so that
+    // their latency overlaps the header's
+    const double *xn = a.xnod + 3 * (size_t)a.ngeo * e;
+    const bool xs_ok = a.ngeo <= XS_MAX;
+    static_assert(3 * XS_MAX <= WG, "one staged node coordinate per thread");
+    const double xv = xs_ok && tid < 3 * a.ngeo ? xn[tid] : 0.0;
+    const double coef = a.coef[e];
+    if (!element_setup<SPACE>(a, e, es, tid)) {
         if (tid == 0 && a.status) a.status[e] = HP3D_E_ORDER;
         return;
     }
@@ -961,14 +969,10 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
         pix[l] = (unsigned)p.l1 | ((unsigned)p.l2 << 4) | ((unsigned)p.l3 << 8);
     }
     // geometry nodes of up to 27-node maps staged in LDS
-    const double *xn = a.xnod + 3 * (size_t)a.ngeo * e;
-    const bool xs_ok = a.ngeo <= XS_MAX;
-    if (xs_ok)
-        for (int k = tid; k < 3 * a.ngeo; k += WG) xg[k] = xn[k];
+    if (xs_ok && tid < 3 * a.ngeo) xg[tid] = xv;
     if (tid == 0) s_fail = 0;
     __syncthreads();
 
-    const double coef = a.coef[e];
     // this thread's row of K's outputs (threads with sub == 0 hold one row each)
     const int row = tid & (R - 1), sub = tid / R;
     const bool row_ok = row < n;
@@ -1324,7 +1328,14 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
 #pragma unroll
         for (int j = 0; j < 4; ++j) da[j] = CY::val(hi[s], acc[s], j);
         d4_t up = d4_t{0, 0, 0, 0};
-        const int ra = R0 + (lane & 15), cb = C0 + (lane & 15);
+        // The FP64 MFMA returns row i of its A operand's rows in register
+        // i / 4 of lane group i % 4, the FP32 accumulator holds row
+        // 4 (l / 16) + j in register j of lane l.  For FP32 tiles, feeding
+        // operand row i = a + 4b (a, b < 4) with tile row 4a + b makes up[j]
+        // of lane l the entry (R0 + 4 (l / 16) + j, C0 + l % 16), da's own
+        // layout; FP64 tiles already share the MFMA's.
+        const int li16 = lane & 15;
+        const int ra = R0 + (sizeof(T) == 4 ? 4 * (li16 & 3) + (li16 >> 2) : li16), cb = C0 + li16;
         const bool rok = ra < ni, cok = cb < ni;
         const int rac = rok ? ra : 0, cbc = cok ? cb : 0;
 #pragma unroll
@@ -1337,15 +1348,8 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
             bw = kok && cok ? bw : 0.0;
             up = MD::mma(av, bw, up);
         }
-        // up[j] is (S^T S)(R0 + MD::row(lane,j), C0 + lane%16): bring it to
-        // the accumulator's row order through this wave's stage
-        double *st = stg + wave * (16 * 17);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) st[MD::row(lane, j) * 17 + (lane & 15)] = up[j];
-        wave_sync_lds();
-#pragma unroll
-        for (int j = 0; j < 4; ++j) da[j] -= st[M::row(lane, j) * 17 + (lane & 15)];
-        wave_sync_lds();
+        for (int j = 0; j < 4; ++j) da[j] -= up[j];
         write_sym_tile_direct<d4_t, double, M>(da, R0, C0, ni, aii, (size_t)ni, lane);
     }
     // X = L_u^-T D^-1/2 S = Abb^-1 [Abi | Bb], stored from the MFMA tiles
@@ -1380,19 +1384,39 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
 template <typename T, int SPACE, int R>
 __global__ __launch_bounds__(WG, R == BR ? 2 : 3) void elem_single_kernel(ElemArgs a, CondArgs cd)
 {
-    single_element<T, SPACE, R>(a, cd, blockIdx.x);
+    single_element<T, SPACE, R>(a, cd, blockIdx.x, threadIdx.x);
 }
+
+// arguments of the persistent form, as one kernel argument at offset 0
+struct SingleArgs {
+    ElemArgs a;
+    CondArgs cd;
+};
+typedef const __attribute__((address_space(4))) SingleArgs *kargs_t;
 
 // persistent form: a fixed grid (2 workgroups per CU) walks the elements with
 // a grid stride; the barrier after each element keeps the next element's
-// setup from overwriting LDS a slower wave still reads
+// setup from overwriting LDS a slower wave still reads.  Per element the
+// arguments are read again from the kernel-argument segment (scalar loads)
+// and the thread index passes through an empty asm, so the compiler keeps
+// nothing derived from them live from one element to the next: the element
+// body needs every register.
 template <typename T, int SPACE, int R>
-__global__ __launch_bounds__(WG, R == BR ? 2 : 3) void elem_single_persistent(ElemArgs a, CondArgs cd)
+__global__ __launch_bounds__(WG, R == BR ? 2 : 3) void elem_single_persistent(SingleArgs sa)
 {
-    for (int e = blockIdx.x; e < a.nel; e += gridDim.x) {
-        single_element<T, SPACE, R>(a, cd, e);
+#if defined(__HIP_DEVICE_COMPILE__)
+    const int nel = sa.a.nel, stride = gridDim.x;
+    for (int e = blockIdx.x; e < nel; e += stride) {
+        kargs_t kp = (kargs_t)__builtin_amdgcn_kernarg_segment_ptr();
+        __asm__ volatile("" : "+s"(kp));
+        const ElemArgs al = kp->a;
+        const CondArgs cl = kp->cd;
+        int tid = threadIdx.x;
+        __asm__ volatile("" : "+v"(tid));
+        single_element<T, SPACE, R>(al, cl, e, tid);
         __syncthreads();
     }
+#endif
 }
 
 // ------------------------------------------------------------ condensation
@@ -2485,12 +2509,13 @@ static int launch_single_r(int space, int precision, const ElemArgs &args, const
         HIPCHK(hipGetDevice(&dev));
         HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
         dim3 grid(std::min(args.nel, (R == BR ? 2 : 3) * std::max(ncu, 1)));
+        const SingleArgs sa{args, cd};
         if (space == HP3D_H1) {
-            if (precision == HP3D_FP64) hipLaunchKernelGGL((elem_single_persistent<double, 0, R>), grid, block, 0, st, args, cd);
-            else hipLaunchKernelGGL((elem_single_persistent<float, 0, R>), grid, block, 0, st, args, cd);
+            if (precision == HP3D_FP64) hipLaunchKernelGGL((elem_single_persistent<double, 0, R>), grid, block, 0, st, sa);
+            else hipLaunchKernelGGL((elem_single_persistent<float, 0, R>), grid, block, 0, st, sa);
         } else {
-            if (precision == HP3D_FP64) hipLaunchKernelGGL((elem_single_persistent<double, 1, R>), grid, block, 0, st, args, cd);
-            else hipLaunchKernelGGL((elem_single_persistent<float, 1, R>), grid, block, 0, st, args, cd);
+            if (precision == HP3D_FP64) hipLaunchKernelGGL((elem_single_persistent<double, 1, R>), grid, block, 0, st, sa);
+            else hipLaunchKernelGGL((elem_single_persistent<float, 1, R>), grid, block, 0, st, sa);
         }
     } else {
         dim3 grid(args.nel);
