@@ -1263,30 +1263,48 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
     // then Bb).  Z = L_u^-1 R, stored as S = D^-1/2 Z; X = L_u^-T (D^-1/2 S).
     const int ntr = (nb + 15) / 16, nct = (ni + 16) / 16, ntile = ntr * nct;
     auto rhs_col = [&](int c) { return c < ni ? c : (c == ni ? n : -1); };
+    // This wave's tiles t = wave + 4u, u < nu (wave-uniform), as independent
+    // MFMA chains interleaved over k: one straight-line body per tile count,
+    // loads from clamped indexes masked at use (no predicated loads)
     auto gemm_li = [&](bool second, d4_t *out) {
+        const int nu = wv < ntile ? (ntile - 1 - wv) / 4 + 1 : 0;
+        auto run = [&](auto nuc) {
+            constexpr int NU = decltype(nuc)::value;
+            int arc[NU], pcc[NU];
+            bool aok[NU], bok[NU];
 #pragma unroll
-        for (int u = 0; u < UMAX; ++u) {
-            int t = wave + 4 * u;
-            out[u] = d4_t{0, 0, 0, 0};
-            if (t >= ntile) continue;
-            int tr = t / nct, tc = t % nct;
-            int ar = tr * 16 + (lane & 15), pc = rhs_col(tc * 16 + (lane & 15));
-            // loads from clamped indexes, masked at use: no predicated
-            // loads, and the fixed trip count lets them run ahead
-            const bool aok = ar < nb, bok = pc >= 0;
-            const int arc = aok ? ar : 0, pcc = bok ? pc : 0;
+            for (int u = 0; u < NU; ++u) {
+                const int t = wv + 4 * u, tr = t / nct, tc = t % nct;
+                const int ar = tr * 16 + (lane & 15), pc = rhs_col(tc * 16 + (lane & 15));
+                aok[u] = ar < nb;
+                bok[u] = pc >= 0;
+                arc[u] = aok[u] ? ar : 0;
+                pcc[u] = bok[u] ? pc : 0;
+                out[u] = d4_t{0, 0, 0, 0};
+            }
 #pragma unroll
             for (int k0 = 0; k0 < NBX; k0 += 4) {
                 const int k = k0 + (lane >> 4);
                 const bool kok = k < nb;
                 const int kc = kok ? k : 0;
-                double av = second ? li[kc * LW + arc] : li[arc * LW + kc];
-                double bw = kb[kc * LDK + pcc];
-                if (second) bw *= rs[kc];
-                av = aok && kok ? av : 0.0;
-                bw = bok && kok ? bw : 0.0;
-                out[u] = MD::mma(av, bw, out[u]);
+                const double rk = second ? rs[kc] : 1.0;
+#pragma unroll
+                for (int u = 0; u < NU; ++u) {
+                    double av = second ? li[kc * LW + arc[u]] : li[arc[u] * LW + kc];
+                    double bw = kb[kc * LDK + pcc[u]] * rk;
+                    av = aok[u] && kok ? av : 0.0;
+                    bw = bok[u] && kok ? bw : 0.0;
+                    out[u] = MD::mma(av, bw, out[u]);
+                }
             }
+        };
+        switch (nu) {
+        case 1: run(std::integral_constant<int, 1>{}); break;
+        case 2: run(std::integral_constant<int, 2>{}); break;
+        case 3: if constexpr (UMAX >= 3) run(std::integral_constant<int, 3>{}); break;
+        case 4: if constexpr (UMAX >= 4) run(std::integral_constant<int, 4>{}); break;
+        case 5: if constexpr (UMAX >= 5) run(std::integral_constant<int, 5>{}); break;
+        default: break;
         }
     };
     auto store_rhs = [&](bool first, const d4_t *in) {
