@@ -367,37 +367,6 @@ __device__ __forceinline__ double load_entry(unsigned d, Pt p, const double (*tv
     return (desc_neg(d) ? -v : v) * gp[18 + desc_comp(d)];
 }
 
-// element header: orders, counts, quadrature sizes (thread 0), plus status on
-// failure.  Returns false (uniformly) if the element is invalid.
-template <int SPACE>
-__device__ __forceinline__ bool element_setup(const ElemArgs &a, int e, ElemShared &es, const int t = threadIdx.x)
-{
-    // the 37 header integers (norder, nori_e, nori_f) in one parallel load
-    __shared__ int hdr[40];
-    if (t < 19) hdr[t] = a.norder[19 * (size_t)e + t];
-    else if (t < 31) hdr[t] = a.nori_e[12 * (size_t)e + (t - 19)];
-    else if (t < 37) hdr[t] = a.nori_f[6 * (size_t)e + (t - 31)];
-    __syncthreads();
-    if (t == 0) {
-        ElemShared s;
-        s.ok = decode_orders(hdr, hdr + 19, hdr + 31, s.o);
-        s.n = s.nb = s.nint = s.nh = 0;
-        s.nq[0] = s.nq[1] = s.nq[2] = 1;
-        if (s.ok) {
-            dof_counts(SPACE, s.o, s.n, s.nb);
-            int nbh;
-            dof_counts(0, s.o, s.nh, nbh);
-            quad_counts(s.o, s.nq);
-            s.nint = s.nq[0] * s.nq[1] * s.nq[2];
-            if (a.ngeo > s.nh) s.ok = 0;
-        }
-        s.negjac = 0;
-        es = s;
-    }
-    __syncthreads();
-    return es.ok != 0;
-}
-
 // Make this wave's LDS writes visible to its other lanes: wait for the LDS
 // counter only (s_waitcnt lgkmcnt(0), vmcnt/expcnt left at their maxima so
 // outstanding global stores are not waited for) and keep the compiler from
@@ -419,6 +388,49 @@ __device__ __forceinline__ void lds_barrier()
     __builtin_amdgcn_s_waitcnt(0xC07F);
     __builtin_amdgcn_s_barrier();
     __asm__ volatile("" ::: "memory");
+}
+
+// element header: orders, counts, quadrature sizes (thread 0), plus status on
+// failure.  Returns false (uniformly) if the element is invalid.
+template <int SPACE>
+__device__ __forceinline__ bool element_setup(const ElemArgs &a, int e, ElemShared &es, const int t = threadIdx.x)
+{
+    // The 37 header integers (norder, nori_e, nori_f) come in through the
+    // scalar cache: wave-uniform loads into SGPRs, which the decoding lane
+    // reads directly.  They are waited for on the scalar counter, so neither
+    // an LDS round trip nor outstanding vector stores (a persistent grid's
+    // previous element) hold them up.
+    typedef const __attribute__((address_space(4))) int *sint_t;
+    if (t < 64) {
+        int hdr[37];
+        const sint_t pn = (sint_t)(uintptr_t)(a.norder + 19 * (size_t)e);
+        const sint_t pe = (sint_t)(uintptr_t)(a.nori_e + 12 * (size_t)e);
+        const sint_t pf = (sint_t)(uintptr_t)(a.nori_f + 6 * (size_t)e);
+#pragma unroll
+        for (int i = 0; i < 19; ++i) hdr[i] = pn[i];
+#pragma unroll
+        for (int i = 0; i < 12; ++i) hdr[19 + i] = pe[i];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) hdr[31 + i] = pf[i];
+        if (t == 0) {
+        ElemShared s;
+        s.ok = decode_orders(hdr, hdr + 19, hdr + 31, s.o);
+        s.n = s.nb = s.nint = s.nh = 0;
+        s.nq[0] = s.nq[1] = s.nq[2] = 1;
+        if (s.ok) {
+            dof_counts(SPACE, s.o, s.n, s.nb);
+            int nbh;
+            dof_counts(0, s.o, s.nh, nbh);
+            quad_counts(s.o, s.nq);
+            s.nint = s.nq[0] * s.nq[1] * s.nq[2];
+            if (a.ngeo > s.nh) s.ok = 0;
+        }
+        s.negjac = 0;
+        es = s;
+        }
+    }
+    lds_barrier();
+    return es.ok != 0;
 }
 
 // A lane's FP64 value read by the whole wave (two 32-bit readlanes; `l` is
