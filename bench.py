@@ -377,8 +377,9 @@ def measure(workload, args, headline):
         forms["batched_fp32_end_to_end"] = {
             "element_matrices_per_s": nel / se2e, "ms_per_step": se2e * 1e3, "d2h_bytes": d2h_bytes,
             "note": "step + D2H of K, load, Aii, Bi, ASchur, BSchur into pinned host memory"}
-        # BF16 trial: bfloat16 operands, FP32 accumulation, block-tiled element
-        # kernel then the blocked condensation
+        # BF16 trial: bfloat16 operands, FP32 accumulation, on the same bins
+        # (fused single-block bins, block-tiled element kernel plus strip
+        # condensation for the larger elements)
         out16 = batch.alloc_outputs(hp3d_amd.BF16)
         s16, _e16 = time_steps(batch, out16, hp3d_amd.BF16, max(2, steps // 2), 1)
         batch.check_status()
@@ -387,7 +388,7 @@ def measure(workload, args, headline):
         del out16
         forms["bf16_trial"] = {"element_matrices_per_s": nel / s16, "ms_per_step": s16 * 1e3,
                                "k_err_max": k16max, "k_err_median": k16med, "aii_err_max": a16max,
-                               "note": "bfloat16 operands, FP32 accumulation (two-pass path)"}
+                               "note": "bfloat16 operands, FP32 accumulation, the FP32 form's bins"}
     achieved = (fk + fc) / (e32 * 1e-3)
     entry = {
         "workload": "%s: %s" % (workload, WORKLOADS[workload]),

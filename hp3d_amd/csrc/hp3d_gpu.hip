@@ -908,10 +908,11 @@ template <typename T> struct SingleCfg {
     static constexpr int KCS = sizeof(T) == 4 ? 48 : 24; // A columns per chunk
 };
 
-template <typename T, int SPACE, int R>
+template <typename T, int SPACE, int R, bool BF = false>
 __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs &cd, const int e,
                                                const int tid)
 {
+    static_assert(!BF || sizeof(T) == 4, "bfloat16 operands accumulate in FP32");
     typedef Mfma<T> M;
     typedef typename M::acc_t acc_t;
     typedef Mfma<double> MD;
@@ -930,7 +931,12 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
     constexpr int LW = NBX + 1;                // leading dimension of L_u^-1
     constexpr int NSLOT = R == BR ? 9 : 3;     // tile slots per wave
     constexpr int UMAX = R == BR ? 5 : 2;      // right-hand-side tiles per wave in the condensation
-    constexpr int BUF = KCS * LD1;             // one A buffer, in T
+    // BF16 trial: the chunk in bfloat16, row-major (a row's KCS values
+    // contiguous, LDB shorts apart), so an MFMA operand (4 consecutive k of
+    // one row) is one 8-byte LDS read
+    constexpr int LDB = KCS + 4;
+    static_assert(!BF || (KCS % 16 == 0 && LDB % 4 == 0), "bf16 chunk layout");
+    constexpr int BUF = BF ? R * LDB / 2 : KCS * LD1; // one A buffer, in T
     constexpr int AC_D = (2 * BUF * (int)sizeof(T) + 7) / 8;
     constexpr int CW_D = NBX * LDK + NBX * LW + NBX + LD1 + 4 * 16 * 17; // output / condensation workspace
     constexpr int SM_D = AC_D > CW_D ? AC_D : CW_D;
@@ -1025,7 +1031,7 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
         slot_tile(s, ti, tj);
         offS[s] = tj * 16 + (lane & 15);
     }
-    typedef Carry<T, acc_t> CY;
+    typedef Carry<std::conditional_t<BF, double, T>, acc_t> CY; // no carry under bfloat16 rounding
     acc_t acc[NSLOT];
     typename CY::hi_t hi[NSLOT];
 #pragma unroll
@@ -1070,7 +1076,12 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
             const bool ok = pok && r < n;
             bacc[i] += ok ? bl : 0.0;
 #pragma unroll
-            for (int c = 0; c < NC; ++c) buf[(NC * qt + c) * LD1 + r] = ok ? (T)out[c] : (T)0;
+            for (int c = 0; c < NC; ++c) {
+                if constexpr (BF)
+                    reinterpret_cast<short *>(buf)[r * LDB + NC * qt + c] = ok ? f2bf((float)out[c]) : (short)0;
+                else
+                    buf[(NC * qt + c) * LD1 + r] = ok ? (T)out[c] : (T)0;
+            }
         }
     };
     // The slot mask is wave-uniform.  When some of this wave's tiles lie beyond
@@ -1081,7 +1092,7 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
     // FP32 H1 with all of a wave's tiles inside N: the loop specialised per
     // wave, so each slot's operand rows are compile-time offsets (no per-MFMA
     // select); the FP64 and H(curl) forms would spill with it
-    constexpr bool SPEC = sizeof(T) == 4 && SPACE == 0;
+    constexpr bool SPEC = sizeof(T) == 4 && SPACE == 0 && !BF;
     // wave W's slots (as set above): rows tiA / tiB, first columns tjA / tjB,
     // nA slots in row tiA, ns slots in all
     auto mfma_w = [&](auto wc, const T *buf) {
@@ -1102,6 +1113,27 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
     // every slot of this wave holds a tile inside N
     const bool wave_live = vmask == (1u << nslot) - 1;
     auto mfma = [&](const T *buf) {
+        if constexpr (BF) {
+            // lane l supplies A[l % 16][4 (l / 16) + i] and B[4 (l / 16) + i][l % 16]
+            // (i < 4) of v_mfma_f32_16x16x16_bf16: row l % 16 of a tile, four
+            // consecutive columns of the chunk
+            const short *sb = reinterpret_cast<const short *>(buf) + (lane & 15) * LDB + 4 * (lane >> 4);
+#pragma unroll
+            for (int k0 = 0; k0 < KCS; k0 += 16) {
+                const s4_t aA = *reinterpret_cast<const s4_t *>(sb + tiA * 16 * LDB + k0);
+                const s4_t aB = *reinterpret_cast<const s4_t *>(sb + tiB * 16 * LDB + k0);
+#pragma unroll
+                for (int s = 0; s < NSLOT; ++s) {
+                    int ti, tj;
+                    slot_tile(s, ti, tj);
+                    if (vmask & (1u << s))
+                        acc[s] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
+                            s < nA ? aA : aB, *reinterpret_cast<const s4_t *>(sb + tj * 16 * LDB + k0), acc[s], 0, 0,
+                            0);
+                }
+            }
+            return;
+        }
         if (SPEC && wave_live) {
             if constexpr (SPEC) {
                 switch (wv) {
@@ -1413,10 +1445,10 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
 // one workgroup per element; R = 64 for elements of at most 64 dofs and 16
 // bubbles: half the LDS and a third of the tile registers, so three
 // workgroups share a CU
-template <typename T, int SPACE, int R>
+template <typename T, int SPACE, int R, bool BF = false>
 __global__ __launch_bounds__(WG, R == BR ? 2 : 3) void elem_single_kernel(ElemArgs a, CondArgs cd)
 {
-    single_element<T, SPACE, R>(a, cd, blockIdx.x, threadIdx.x);
+    single_element<T, SPACE, R, BF>(a, cd, blockIdx.x, threadIdx.x);
 }
 
 // arguments of the persistent form, as one kernel argument at offset 0
@@ -2536,7 +2568,11 @@ static int launch_single_r(int space, int precision, const ElemArgs &args, const
                            hipStream_t st)
 {
     dim3 block(WG);
-    if (g_persistent) {
+    if (precision == HP3D_BF16) {
+        dim3 grid(args.nel);
+        if (space == HP3D_H1) hipLaunchKernelGGL((elem_single_kernel<float, 0, R, true>), grid, block, 0, st, args, cd);
+        else hipLaunchKernelGGL((elem_single_kernel<float, 1, R, true>), grid, block, 0, st, args, cd);
+    } else if (g_persistent) {
         int dev = 0, ncu = 0;
         HIPCHK(hipGetDevice(&dev));
         HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
@@ -2576,7 +2612,7 @@ static int launch_single(int space, int precision, const ElemArgs &args, const C
 static int launch_elem(int space, int precision, const ElemArgs &args, int max_ndof,
                        hipStream_t st)
 {
-    if (max_ndof <= BR && precision != HP3D_BF16) {
+    if (max_ndof <= BR) {
         CondArgs cd{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
         return launch_single(space, precision, args, cd, max_ndof, 0, st);
     }
@@ -2801,7 +2837,7 @@ int hp3d_gpu_elem_batch(int space, int precision, int nel, const int *norder, co
 size_t hp3d_gpu_elem_workspace(int space, int precision, int nel, int max_ndof)
 {
     if (nel <= 0 || max_ndof <= 0 || (space != HP3D_H1 && space != HP3D_HCURL)) return 0;
-    if (max_ndof <= BR && precision != HP3D_BF16) return 0; // single-block kernel
+    if (max_ndof <= BR) return 0; // single-block kernel
     return (size_t)nel * geo_slab(space) * sizeof(double);
 }
 
@@ -2813,7 +2849,7 @@ size_t hp3d_gpu_stc_workspace(int nel, int max_ndof, int max_nbubble)
 
 size_t hp3d_gpu_celem_workspace(int space, int precision, int nel, int max_ndof, int max_nbubble)
 {
-    if (max_ndof <= BR && max_nbubble <= NBF && precision != HP3D_BF16) return 0; // fused
+    if (max_ndof <= BR && max_nbubble <= NBF) return 0; // fused
     // the geometry pass and the condensation use the same memory one after the other
     return std::max(hp3d_gpu_elem_workspace(space, precision, nel, max_ndof),
                     hp3d_gpu_stc_workspace(nel, max_ndof, max_nbubble));
@@ -2904,7 +2940,7 @@ int hp3d_gpu_celem_batch(int space, int precision, int nel, const int *norder, c
     a.status = status;
     a.gws = (double *)workspace;
     hipStream_t st = (hipStream_t)stream;
-    if (max_ndof <= BR && max_nbubble <= NBF && precision != HP3D_BF16) {
+    if (max_ndof <= BR && max_nbubble <= NBF) {
         CondArgs cd{off_c, off_s, aii, bi, aschur, bschur};
         return launch_single(space, precision, a, cd, max_ndof, max_nbubble, st);
     }

@@ -265,8 +265,7 @@ class ElementBatch:
         return ws
 
     def _ensure_workspace(self, precision):
-        """Grow the workspace for `precision` (the BF16 trial runs every bin
-        through the block-tiled kernel); never called during a capture."""
+        """Grow the workspace for `precision`; never called during a capture."""
         import torch
 
         need = self._workspace_bytes(precision)
@@ -294,9 +293,8 @@ class ElementBatch:
         dt = torch.float64 if precision == FP64 else torch.float32
         out = {}
         dev = self.device
-        # two-pass bins (and the BF16 trial, which has no single-block form)
-        # condense from K in HBM
-        if want_k or (want_stc and ("two_pass" in self.bin_paths or precision == BF16)):
+        # two-pass bins condense from K in HBM
+        if want_k or (want_stc and "two_pass" in self.bin_paths):
             out["k"] = torch.empty(self.size_k, dtype=dt, device=dev)
             out["b"] = torch.empty(self.size_v, dtype=torch.float64, device=dev)
         if want_stc:
@@ -351,8 +349,7 @@ class ElementBatch:
         current torch stream) the fused bins and the two-pass bins run on two
         side streams forked from and joined back to the current stream, so the
         tail of one launch overlaps the other group's work; the bins write
-        disjoint outputs and only the two-pass bins use the workspace (so the
-        BF16 trial, which runs every bin two-pass, stays sequential)."""
+        disjoint outputs and only the two-pass bins use the workspace."""
         import torch
 
         self._ensure_workspace(precision)
@@ -362,8 +359,7 @@ class ElementBatch:
         # against 2.31 ms): the fused kernels pack worse beside each other
         groups = [g for g in ([i for i, p in enumerate(self.bin_paths) if p == "fused"],
                               [i for i, p in enumerate(self.bin_paths) if p == "two_pass"]) if g]
-        # under BF16 every bin runs two-pass through the workspace: sequential
-        if concurrent and stream is None and len(groups) > 1 and precision != BF16:
+        if concurrent and stream is None and len(groups) > 1:
             main = torch.cuda.current_stream(self.device)
             side = getattr(self, "_side", [])
             while len(side) < len(groups):

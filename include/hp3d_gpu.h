@@ -143,7 +143,7 @@ int hp3d_gpu_celem(int space, const int norder[19], const int nori_e[12], const 
  * max_ndof bounds N_e over the batch (it sizes the launch grid).
  * workspace: device memory of hp3d_gpu_elem_workspace(space, precision, nel,
  * max_ndof) bytes (the per-element quadrature-point geometry of the
- * block-tiled kernel, N_e > 128 or BF16); may be NULL when that size is 0.
+ * block-tiled kernel, N_e > 128); may be NULL when that size is 0.
  * Replaces the element loop of mumps_sc Step 2 up to elem
  * (mumps_sc.F90:253-259 -> celem_systemI.F90:523).
  */
