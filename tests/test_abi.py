@@ -89,7 +89,8 @@ def test_entries_before_init_and_bad_arguments():
     # geometry slabs: one per element of the block-tiled path (343 points x 10 or
     # 21 doubles + a flag), none for the single-block kernel
     assert lib.hp3d_gpu_elem_workspace(0, 1, 10, 128) == 0
-    assert lib.hp3d_gpu_elem_workspace(0, 2, 10, 27) == 10 * (343 * 10 + 2) * 8
+    assert lib.hp3d_gpu_elem_workspace(0, 2, 10, 27) == 0  # BF16 trial: single-block too
+    assert lib.hp3d_gpu_elem_workspace(0, 2, 10, 200) == 10 * (343 * 10 + 2) * 8
     assert lib.hp3d_gpu_elem_workspace(1, 0, 10, 540) == 10 * (343 * 21 + 2) * 8
     assert lib.hp3d_gpu_celem_workspace(0, 1, 10, 125, 27) == 0
     assert lib.hp3d_gpu_celem_workspace(1, 1, 10, 540, 240) == max(
