@@ -57,8 +57,9 @@ extern "C" {
  * bubble factorisation are always FP64 */
 #define HP3D_FP64 0
 #define HP3D_FP32 1
-/* trial: operands rounded to bfloat16, FP32 accumulation, K stored as FP32
- * (block-tiled element kernel only; condensation as for FP32) */
+/* trial: operands rounded to bfloat16, FP32 accumulation (no carry), K stored
+ * as FP32; the same kernels as FP32 (single-block and block-tiled), and the
+ * condensation as for FP32 */
 #define HP3D_BF16 2
 
 /* load selectors (the user getf routine) */
