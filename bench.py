@@ -317,7 +317,7 @@ def measure(workload, args, headline):
 
         rules, tuning = tune.autotune(space, m, hp3d_amd.FP32, reps=3)
     batch = hp3d_amd.ElementBatch.from_mesh(space, m, rules=rules)
-    steps = args.steps if headline else max(3, args.steps // 2)
+    steps = args.steps
     out32 = batch.alloc_outputs(hp3d_amd.FP32)
     out64 = batch.alloc_outputs(hp3d_amd.FP64)
 
