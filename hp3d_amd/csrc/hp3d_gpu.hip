@@ -247,7 +247,7 @@ __device__ __forceinline__ void point_map(Pt p, const int *nq, const double (*tv
 // geometry of one quadrature point (FP64): x, J, det J, J^-1 folded with the
 // weights into GS doubles at gp (geom3D.F90:77-91, geom.F90:57-113,
 // elem_opt.F90:101-122).  Returns true when det J < 0.
-template <int SPACE>
+template <int SPACE, bool SPLIT = false>
 __device__ __forceinline__ bool point_geometry(const ElemShared &es, int l, const double (*tv)[NTAB * NPT],
                                const double (*td)[NTAB * NPT], const unsigned *dgeo,
                                const double *xn, int ngeo, double coef, int load, double *gp)
@@ -255,14 +255,22 @@ __device__ __forceinline__ bool point_geometry(const ElemShared &es, int l, cons
     double x[3], jm[3][3], inv[3][3], det, w;
     point_map(point_ix(l, es.nq), es.nq, tv, td, dgeo, xn, ngeo, x, jm, inv, det, w);
     double wd = det * w;
-    double f[3];
-    getf(load, x, f);
     if (SPACE == 0) {
         double s = sqrt(coef * wd);
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j) gp[3 * i + j] = inv[i][j] * s; // [ref][phys]
-        gp[9] = f[0] * wd;
-    } else {
+        if (SPLIT) {
+            gp[9] = wd; // the load f(x) comes from point_load_h1 into gp[10]
+        } else {
+            double f[3];
+            getf(load, x, f);
+            gp[9] = f[0] * wd;
+        }
+        return det < 0.0;
+    }
+    double f[3];
+    getf(load, x, f);
+    {
         double sc = sqrt(wd) / det, sm = sqrt(coef * wd); // mu = 1
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j) {
@@ -273,6 +281,26 @@ __device__ __forceinline__ bool point_geometry(const ElemShared &es, int l, cons
             gp[18 + c] = (inv[c][0] * f[0] + inv[c][1] * f[1] + inv[c][2] * f[2]) * wd;
     }
     return det < 0.0;
+}
+
+// The scalar load f(x) of the H1 form at point l: x summed over the geometry
+// nodes in point_map's order (so it is the same x), then getf.  Lets the
+// 64-row fused kernel evaluate the load on another wave than the Jacobian.
+__device__ __forceinline__ double point_load_h1(const ElemShared &es, int l, const double (*tv)[NTAB * NPT],
+                                                const unsigned *dgeo, const double *xn, int ngeo, int load)
+{
+    const Pt p = point_ix(l, es.nq);
+    double x[3] = {0.0, 0.0, 0.0};
+    for (int k = 0; k < ngeo; ++k) {
+        unsigned d = dgeo[k];
+        int i0 = desc_ix(d, 0), i1 = desc_ix(d, 1), i2 = desc_ix(d, 2);
+        double v0 = tv[0][p.l1 * NTAB + i0], v1 = tv[1][p.l2 * NTAB + i1], v2 = tv[2][p.l3 * NTAB + i2];
+        double ph = v0 * v1 * v2;
+        for (int c = 0; c < 3; ++c) x[c] += xn[3 * k + c] * ph;
+    }
+    double f[3];
+    getf(load, x, f);
+    return f[0];
 }
 
 // for the H(curl) component cp (0..2) with sign sg and the reference gradient
@@ -922,7 +950,11 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
     // points per geometry pass (a multiple of QC): 128 for the 128-row H1
     // form, 64 where a third workgroup per CU has to fit in LDS
     constexpr int GQ = SPACE == 0 && R == BR ? 128 : 64;
-    constexpr int GS = SPACE == 0 ? 10 : 21;
+    // 64-row H1 form: the Jacobian (wave 0) and the load f(x) (wave 2) of a
+    // point are evaluated side by side; gp[9] = w det J, gp[10] = f
+    constexpr bool SPLIT = SPACE == 0 && R == BR / 2;
+    static_assert(!SPLIT || 2 * GQ <= WG, "load threads beside the geometry threads");
+    constexpr int GS = SPACE == 0 ? (SPLIT ? 11 : 10) : 21;
     static_assert(R == BR || R == BR / 2, "128 or 64 rows");
     constexpr int LD1 = R + 16;                // LDS stride of an A column (in T)
     constexpr int LDK = LD1 + 1;               // LDS stride of a bubble row (doubles; odd, so
@@ -1068,6 +1100,7 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
         double g[GS];
 #pragma unroll
         for (int k = 0; k < GS; ++k) g[k] = geo[lqc * GS + k];
+        if (SPLIT) g[9] = g[10] * g[9]; // f (w det J), as the unsplit pass forms it
 #pragma unroll 4
         for (int i = 0; i < RPT; ++i) {
             const int r = rt + TPP * i;
@@ -1169,11 +1202,15 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
     for (int g0 = 0; g0 < nint; g0 += GQ) {
         const int gend = min(GQ, nint - g0);
         if (tid < gend) {
-            const bool neg = xs_ok ? point_geometry<SPACE>(es, g0 + tid, tv, td, dgeo, xg, a.ngeo, coef,
-                                                           a.load, geo + tid * GS)
-                                   : point_geometry<SPACE>(es, g0 + tid, tv, td, dgeo, xn, a.ngeo, coef,
-                                                           a.load, geo + tid * GS);
+            const bool neg = xs_ok ? point_geometry<SPACE, SPLIT>(es, g0 + tid, tv, td, dgeo, xg, a.ngeo, coef,
+                                                                  a.load, geo + tid * GS)
+                                   : point_geometry<SPACE, SPLIT>(es, g0 + tid, tv, td, dgeo, xn, a.ngeo, coef,
+                                                                  a.load, geo + tid * GS);
             if (neg) es.negjac = 1;
+        } else if (SPLIT && tid >= WG / 2 && tid - WG / 2 < gend) {
+            const int l = tid - WG / 2;
+            geo[l * GS + 10] = xs_ok ? point_load_h1(es, g0 + l, tv, dgeo, xg, a.ngeo, a.load)
+                                     : point_load_h1(es, g0 + l, tv, dgeo, xn, a.ngeo, a.load);
         }
         __syncthreads();
         const int nch = (gend + QC - 1) / QC;
