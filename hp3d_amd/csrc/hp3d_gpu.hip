@@ -1122,9 +1122,12 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
     // unguarded loop keeps the MFMAs back to back.
     const unsigned vmask = __builtin_amdgcn_readfirstlane(valid);
     const bool all_live = vmask == (1u << NSLOT) - 1;
-    // FP32 H1 with all of a wave's tiles inside N: the loop specialised per
-    // wave, so each slot's operand rows are compile-time offsets (no per-MFMA
-    // select); the FP64 and H(curl) forms would spill with it
+    // FP32 H1: the loop specialised per wave, so each slot's operand rows are
+    // compile-time offsets (no per-MFMA select or branch); the FP64 and
+    // H(curl) forms would spill with it.  It runs every slot of the wave also
+    // when some tiles lie beyond N (N <= 112, or 48 in the 64-row form): the
+    // build stores zeros in the rows from N to R, such tiles come out zero,
+    // and no output or condensation step reads a slot outside `valid`
     constexpr bool SPEC = sizeof(T) == 4 && SPACE == 0 && !BF;
     // wave W's slots (as set above): rows tiA / tiB, first columns tjA / tjB,
     // nA slots in row tiA, ns slots in all
@@ -1143,8 +1146,6 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
         }
     };
     const int wv = __builtin_amdgcn_readfirstlane(wave);
-    // every slot of this wave holds a tile inside N
-    const bool wave_live = vmask == (1u << nslot) - 1;
     auto mfma = [&](const T *buf) {
         if constexpr (BF) {
             // lane l supplies A[l % 16][4 (l / 16) + i] and B[4 (l / 16) + i][l % 16]
@@ -1167,7 +1168,7 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
             }
             return;
         }
-        if (SPEC && wave_live) {
+        if (SPEC) {
             if constexpr (SPEC) {
                 switch (wv) {
                 case 0: mfma_w(std::integral_constant<int, 0>{}, buf); break;
