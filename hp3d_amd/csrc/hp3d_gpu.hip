@@ -1146,6 +1146,44 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
         }
     };
     const int wv = __builtin_amdgcn_readfirstlane(wave);
+    // The FP32 H1 chunk step with the next chunk's build woven in: after each
+    // k step's MFMAs, one row of the next chunk (rows during the first RPT
+    // k steps), so that the build's VALU and LDS work can issue while the
+    // MFMAs run
+    auto mfma_build_w = [&](auto wc, const T *buf, int g0, int l0, int gend, T *nbuf) {
+        constexpr int W = decltype(wc)::value, P = W & 1;
+        constexpr int TA = R == BR ? W : P, TB = R == BR ? 7 - W : 3 - P;
+        constexpr int JA = R == BR ? W : (W < 2 ? P : P + 3), JB = TB;
+        constexpr int NA = R == BR ? 8 - W : (W < 2 ? 3 : 1 - P), NS = R == BR ? 9 : (W < 2 ? 3 : 2);
+        constexpr int NKS = KCS / 4;
+        static_assert(RPT <= NKS, "a row per k step");
+        const int lq = l0 + qt;
+        const bool pok = lq < gend;
+        const int lqc = pok ? lq : 0;
+        const unsigned pk = pix[g0 + lqc];
+        const Pt p{(int)(pk & 15), (int)((pk >> 4) & 15), (int)(pk >> 8)};
+        double g[GS];
+#pragma unroll
+        for (int k = 0; k < GS; ++k) g[k] = geo[lqc * GS + k];
+        if (SPLIT) g[9] = g[10] * g[9];
+#pragma unroll
+        for (int ks = 0; ks < NKS; ++ks) {
+            const T *col = buf + (4 * ks + (lane >> 4)) * LD1 + (lane & 15);
+            const T aA = col[TA * 16], aB = col[TB * 16];
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+                acc[s] = M::mma(s < NA ? aA : aB, col[(s < NA ? JA + s : JB + (s - NA)) * 16], acc[s]);
+            if (ks < RPT) {
+                const int r = rt + TPP * ks;
+                double out[NC];
+                const double bl = a_entries_load<SPACE, true>(dsp[r], p, tv, td, g, out);
+                const bool ok = pok && r < n;
+                bacc[ks] += ok ? bl : 0.0;
+#pragma unroll
+                for (int c = 0; c < NC; ++c) nbuf[(NC * qt + c) * LD1 + r] = ok ? (T)out[c] : (T)0;
+            }
+        }
+    };
     auto mfma = [&](const T *buf) {
         if constexpr (BF) {
             // lane l supplies A[l % 16][4 (l / 16) + i] and B[4 (l / 16) + i][l % 16]
@@ -1218,8 +1256,20 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
         build(g0, 0, gend, ac);
         __syncthreads();
         for (int c = 0; c < nch; ++c) {
-            mfma(ac + (c & 1) * BUF);
-            if (c + 1 < nch) build(g0, (c + 1) * QC, gend, ac + ((c + 1) & 1) * BUF);
+            T *const cur = ac + (c & 1) * BUF, *const nxt = ac + ((c + 1) & 1) * BUF;
+            if (SPEC && c + 1 < nch) {
+                if constexpr (SPEC) {
+                    switch (wv) {
+                    case 0: mfma_build_w(std::integral_constant<int, 0>{}, cur, g0, (c + 1) * QC, gend, nxt); break;
+                    case 1: mfma_build_w(std::integral_constant<int, 1>{}, cur, g0, (c + 1) * QC, gend, nxt); break;
+                    case 2: mfma_build_w(std::integral_constant<int, 2>{}, cur, g0, (c + 1) * QC, gend, nxt); break;
+                    default: mfma_build_w(std::integral_constant<int, 3>{}, cur, g0, (c + 1) * QC, gend, nxt); break;
+                    }
+                }
+            } else {
+                mfma(cur);
+                if (c + 1 < nch) build(g0, (c + 1) * QC, gend, nxt);
+            }
             if ((++nfold & (FOLD_EVERY - 1)) == 0) {
 #pragma unroll
                 for (int s = 0; s < NSLOT; ++s) CY::fold(hi[s], acc[s]);
