@@ -1149,7 +1149,8 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
     // The FP32 H1 chunk step with the next chunk's build woven in: after each
     // k step's MFMAs, one row of the next chunk (rows during the first RPT
     // k steps), so that the build's VALU and LDS work can issue while the
-    // MFMAs run
+    // MFMAs run.  The 128-row form only: the 64-row form's chunk has a third
+    // of the MFMAs to hide the build under, and measured slower with it.
     auto mfma_build_w = [&](auto wc, const T *buf, int g0, int l0, int gend, T *nbuf) {
         constexpr int W = decltype(wc)::value, P = W & 1;
         constexpr int TA = R == BR ? W : P, TB = R == BR ? 7 - W : 3 - P;
@@ -1257,8 +1258,8 @@ __device__ __forceinline__ void single_element(const ElemArgs &a, const CondArgs
         __syncthreads();
         for (int c = 0; c < nch; ++c) {
             T *const cur = ac + (c & 1) * BUF, *const nxt = ac + ((c + 1) & 1) * BUF;
-            if (SPEC && c + 1 < nch) {
-                if constexpr (SPEC) {
+            if (SPEC && R == BR && c + 1 < nch) {
+                if constexpr (SPEC && R == BR) {
                     switch (wv) {
                     case 0: mfma_build_w(std::integral_constant<int, 0>{}, cur, g0, (c + 1) * QC, gend, nxt); break;
                     case 1: mfma_build_w(std::integral_constant<int, 1>{}, cur, g0, (c + 1) * QC, gend, nxt); break;
