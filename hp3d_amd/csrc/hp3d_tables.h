@@ -30,7 +30,7 @@ HP3D_HD constexpr int vtx_mu(int v, int d)
 // edge e: direction of its coordinate, and the (affine index, direction) of
 // its two blending factors
 struct EdgeT { int dir, a1, d1, a2, d2; };
-HP3D_HD inline EdgeT edge_t(int e)
+HP3D_HD constexpr EdgeT edge_t(int e)
 {
     switch (e) {
     case 0: return {0, 0, 1, 0, 2};
@@ -51,7 +51,7 @@ HP3D_HD inline EdgeT edge_t(int e)
 // face f: (affine index, direction) of its blending factor and its two local
 // axes (element directions of local s and t)
 struct FaceT { int a, dn, ax0, ax1; };
-HP3D_HD inline FaceT face_t(int f)
+HP3D_HD constexpr FaceT face_t(int f)
 {
     switch (f) {
     case 0: return {0, 2, 0, 1};
@@ -66,7 +66,7 @@ HP3D_HD inline FaceT face_t(int f)
 // face orientation code n in 0..7 -> which local axis becomes the face-node
 // s and t axes, and whether each is reversed
 struct FaceOri { int ps, pt, rs, rt; };
-HP3D_HD inline FaceOri face_ori(int n)
+HP3D_HD constexpr FaceOri face_ori(int n)
 {
     // (OrPa(1)-1, OrPa(2)-1, OrGS(1)==0, OrGT(1)==0)
     switch (n) {
