@@ -60,10 +60,24 @@ __constant__ double c_gw[10][10] = HP3D_GAUSS_W;
 // 1D factor tables at the Gauss points of the three directions (values and
 // d/dxi), one work item per (direction, point, flip) so that one wave fills
 // all of them: 2 + 2 (MAXP - 1) + 2 MAXP entries per (direction, point)
+// Gauss node q of the n-point rule by selects over compile-time constants:
+// no memory load on the table fill's critical path
+__device__ __forceinline__ double gauss_x_sel(int n, int q)
+{
+    constexpr double gx[10][10] = HP3D_GAUSS_X;
+    const int key = 16 * n + q;
+    double v = 0.0;
+#pragma unroll
+    for (int nn = 1; nn <= 10; ++nn)
+#pragma unroll
+        for (int qq = 0; qq < nn; ++qq) v = key == 16 * nn + qq ? gx[nn - 1][qq] : v;
+    return v;
+}
+
 __device__ __forceinline__ void fill_table_item(int d, int q, int flip, int n, double (*tv)[NTAB * NPT],
                                                 double (*td)[NTAB * NPT])
 {
-    double x = 0.5 * (1.0 + c_gx[n - 1][q]);
+    double x = 0.5 * (1.0 + gauss_x_sel(n, q));
     if (flip == 0) {
         tv[d][q * NTAB + 0] = 1.0 - x;
         td[d][q * NTAB + 0] = -1.0;
@@ -74,12 +88,16 @@ __device__ __forceinline__ void fill_table_item(int d, int q, int flip, int n, d
     // d phi_i / ds = P_{i-1}(2s-1)   (PolyILegendre, Idec = .true.)
     double s = flip ? 1.0 - x : x;
     double y = 2.0 * s - 1.0, p[MAXP + 1];
+    // the divisions by the small integers as products with their
+    // (compile-time) reciprocals: within an ulp of the quotients
     p[0] = 1.0;
     p[1] = y;
+#pragma unroll
     for (int i = 2; i <= MAXP; ++i)
-        p[i] = ((double)(2 * i - 1) * y * p[i - 1] - (double)(i - 1) * p[i - 2]) / (double)i;
+        p[i] = ((double)(2 * i - 1) * y * p[i - 1] - (double)(i - 1) * p[i - 2]) * (1.0 / (double)i);
+#pragma unroll
     for (int i = 2; i <= MAXP; ++i) {
-        tv[d][q * NTAB + t_phi(i, flip)] = (p[i] - p[i - 2]) / (double)(4 * i - 2);
+        tv[d][q * NTAB + t_phi(i, flip)] = (p[i] - p[i - 2]) * (1.0 / (double)(4 * i - 2));
         td[d][q * NTAB + t_phi(i, flip)] = flip ? -p[i - 1] : p[i - 1];
     }
     // scaled Legendre P_i(s; t) with t = s0 + s1 evaluated as the reference
@@ -88,8 +106,9 @@ __device__ __forceinline__ void fill_table_item(int d, int q, int flip, int n, d
     double yy = 2.0 * s - t, tt = t * t, r[MAXP];
     r[0] = 1.0;
     r[1] = yy;
+#pragma unroll
     for (int i = 2; i < MAXP; ++i)
-        r[i] = ((double)(2 * i - 1) * yy * r[i - 1] - (double)(i - 1) * tt * r[i - 2]) / (double)i;
+        r[i] = ((double)(2 * i - 1) * yy * r[i - 1] - (double)(i - 1) * tt * r[i - 2]) * (1.0 / (double)i);
     for (int i = 0; i < MAXP; ++i) {
         tv[d][q * NTAB + t_leg(i, flip)] = r[i];
         td[d][q * NTAB + t_leg(i, flip)] = 0.0;
