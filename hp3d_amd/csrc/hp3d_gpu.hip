@@ -946,101 +946,6 @@ __global__ __launch_bounds__(WGE, 4) void elem_kernel(ElemArgs a)
 // The A chunks are double buffered: the MFMAs of chunk c are issued before
 // the VALU/LDS work that builds chunk c+1, so the two overlap.
 constexpr int NBF = 32;      // largest bubble block condensed in the fused path
-// Descriptor pass of the fused kernels: desc_h1 with the dof's entity found
-// by a branch-free scan over the edge and face dof counts (every lane runs the
-// same instructions; the reference walk exits at a different entity per lane,
-// so a wave ran every exit), the edge, face and orientation tables as packed
-// constants, and the face and bubble index splits by a float reciprocal, so a
-// wave takes each entity kind's path once.  Same descriptors as desc_h1.
-constexpr unsigned long long pack_edges(int lo)
-{
-    unsigned long long w = 0;
-    for (int e = lo; e < lo + 8 && e < 12; ++e) {
-        const EdgeT t = edge_t(e);
-        w |= (unsigned long long)(t.dir | t.a1 << 2 | t.d1 << 3 | t.a2 << 5 | t.d2 << 6) << (8 * (e - lo));
-    }
-    return w;
-}
-constexpr unsigned long long pack_faces()
-{
-    unsigned long long w = 0;
-    for (int f = 0; f < 6; ++f) {
-        const FaceT t = face_t(f);
-        w |= (unsigned long long)(t.a | t.dn << 1 | t.ax0 << 3 | t.ax1 << 5) << (8 * f);
-    }
-    return w;
-}
-constexpr unsigned pack_face_ori()
-{
-    unsigned w = 0;
-    for (int c = 0; c < 8; ++c) {
-        const FaceOri t = face_ori(c);
-        w |= (unsigned)(t.ps | t.pt << 1 | t.rs << 2 | t.rt << 3) << (4 * c);
-    }
-    return w;
-}
-constexpr unsigned long long EDGE_LO = pack_edges(0), EDGE_HI = pack_edges(8), FACES = pack_faces();
-constexpr unsigned FACE_ORI = pack_face_ori();
-
-// floor(k / d) for 0 <= k < 1024, 1 <= d <= 36: (k + 1/2) / d lies at least
-// 1/72 from an integer, far beyond the reciprocal's rounding
-__device__ __forceinline__ int div_small(int k, int d)
-{
-    return (int)(((float)k + 0.5f) * __builtin_amdgcn_rcpf((float)d));
-}
-
-__device__ __forceinline__ unsigned desc_h1_scan(int k, const ElemOrders &o)
-{
-    if (k < 8) return pack_desc(vtx_mu(k, 0), vtx_mu(k, 1), vtx_mu(k, 2), 0, 0);
-    k -= 8;
-    // the entity's orientation and face width are picked up during the scan,
-    // so nothing is indexed by a lane-dependent value
-    int acc = 0, ent = -1, kl = 0, ori = 0, h1 = 1;
-#pragma unroll
-    for (int e = 0; e < 12; ++e) {
-        const int ne = o.pe[e] - 1;
-        const bool hit = ent < 0 && k < acc + ne;
-        ent = hit ? e : ent;
-        kl = hit ? k - acc : kl;
-        ori = hit ? o.oe[e] : ori;
-        acc += ne;
-    }
-#pragma unroll
-    for (int f = 0; f < 6; ++f) {
-        const int nf = (o.fh[f] - 1) * (o.fv[f] - 1);
-        const bool hit = ent < 0 && k < acc + nf;
-        ent = hit ? 12 + f : ent;
-        kl = hit ? k - acc : kl;
-        ori = hit ? o.of[f] : ori;
-        h1 = hit ? o.fh[f] - 1 : h1;
-        acc += nf;
-    }
-    // (direction, index) pairs that together cover the three directions
-    auto put = [](int d0, int v0, int d1, int v1, int d2, int v2) {
-        const int i0 = d0 == 0 ? v0 : (d1 == 0 ? v1 : v2);
-        const int i1 = d0 == 1 ? v0 : (d1 == 1 ? v1 : v2);
-        const int i2 = d0 == 2 ? v0 : (d1 == 2 ? v1 : v2);
-        return pack_desc(i0, i1, i2, 0, 0);
-    };
-    if (ent < 0) { // bubble
-        const int kb = k - acc, bx = o.b[0] - 1, by = o.b[1] - 1;
-        const int q1 = div_small(kb, bx), q2 = div_small(kb, bx * by);
-        return pack_desc(kb - q1 * bx + 2, q1 - div_small(q1, by) * by + 2, q2 + 2, 0, 0);
-    }
-    if (ent < 12) { // edge
-        const unsigned b = (unsigned)(((ent < 8 ? EDGE_LO : EDGE_HI) >> (8 * (ent & 7))) & 0xffu);
-        const int dir = b & 3, a1 = (b >> 2) & 1, d1 = (b >> 3) & 3, a2 = (b >> 5) & 1, d2 = (b >> 6) & 3;
-        return put(dir, t_phi(kl + 2, ori), d1, a1, d2, a2);
-    }
-    const int f = ent - 12; // face
-    const unsigned b = (unsigned)((FACES >> (8 * f)) & 0xffu);
-    const int fa = b & 1, dn = (b >> 1) & 3, ax0 = (b >> 3) & 3, ax1 = (b >> 5) & 3;
-    const unsigned oo = (FACE_ORI >> (4 * ori)) & 15u;
-    const int ps = oo & 1, pt = (oo >> 1) & 1, rs = (oo >> 2) & 1, rt = (oo >> 3) & 1;
-    const int j0 = div_small(kl, h1);
-    return put(ps ? ax1 : ax0, t_phi(kl - j0 * h1 + 2, rs), pt ? ax1 : ax0, t_phi(j0 + 2, rt), dn, fa);
-}
-
 struct CondArgs {
     const long long *off_c, *off_s;
     double *aii, *bi, *aschur, *bschur;
