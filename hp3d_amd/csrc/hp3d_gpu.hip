@@ -462,20 +462,20 @@ __device__ __forceinline__ bool element_setup(const ElemArgs &a, int e, ElemShar
 #pragma unroll
         for (int i = 0; i < 6; ++i) hdr[31 + i] = pf[i];
         if (t == 0) {
-        ElemShared s;
-        s.ok = decode_orders(hdr, hdr + 19, hdr + 31, s.o);
-        s.n = s.nb = s.nint = s.nh = 0;
-        s.nq[0] = s.nq[1] = s.nq[2] = 1;
-        if (s.ok) {
-            dof_counts(SPACE, s.o, s.n, s.nb);
-            int nbh;
-            dof_counts(0, s.o, s.nh, nbh);
-            quad_counts(s.o, s.nq);
-            s.nint = s.nq[0] * s.nq[1] * s.nq[2];
-            if (a.ngeo > s.nh) s.ok = 0;
-        }
-        s.negjac = 0;
-        es = s;
+            ElemShared s;
+            s.ok = decode_orders(hdr, hdr + 19, hdr + 31, s.o);
+            s.n = s.nb = s.nint = s.nh = 0;
+            s.nq[0] = s.nq[1] = s.nq[2] = 1;
+            if (s.ok) {
+                dof_counts(SPACE, s.o, s.n, s.nb);
+                int nbh;
+                dof_counts(0, s.o, s.nh, nbh);
+                quad_counts(s.o, s.nq);
+                s.nint = s.nq[0] * s.nq[1] * s.nq[2];
+                if (a.ngeo > s.nh) s.ok = 0;
+            }
+            s.negjac = 0;
+            es = s;
         }
     }
     lds_barrier();
